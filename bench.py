@@ -1,0 +1,194 @@
+"""Benchmark of miniFE's assemble-and-solve path on one MI355X.
+
+Workload: the 300^3 hex8 box problem (27,270,901 rows, 731,432,701 non-zeros,
+FP64 values, int32 indices), assembled on the GPU, then plain FP64 CG with 200
+iterations (the reference's max_iters, driver.hpp:241).  A "step" is one
+200-iteration CG solve from x = 0 on the resident matrix; `value` is CG
+iterations per second over the timed steps.
+
+Also reported (not part of `value`): the SpMV kernel's HBM rate and roofline
+fraction (HIP events around back-to-back launches of the exact kernel the CG
+runs), assembly rate (elements/s), time to ||r||/||b|| <= 1e-10 in FP64, and a
+CPU baseline: the oracle's serial restatement of the reference CG timed on this
+host over a bounded sample of the same matrix.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 300]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def spmv_bytes(n: int, nnz: int, value_bytes: int = 8) -> int:
+    """Algorithmic HBM bytes of one CSR SpMV: values + int32 columns, row
+    offsets, x read once and y written once."""
+    return (value_bytes + 4) * nnz + 4 * (n + 1) + 2 * value_bytes * n
+
+
+def cpu_baseline(A_dev, b_dev, n, nnz, iters: int):
+    """Serial oracle CG (restatement of ref/ cg_solve + matvec_std) on the same
+    300^3 matrix, `iters` iterations; scaled to iterations/s."""
+    import oracle
+    if not os.path.exists(oracle.LIB_PATH):
+        return None
+    h = A_dev.to_host(rows=False)
+    A = oracle.CSR(np.arange(n, dtype=np.int32), h["row_offsets"], h["cols"], h["vals"])
+    b = b_dev.to_numpy()
+    del h
+    t0 = time.perf_counter()
+    r = oracle.cg_solve(A, b, np.zeros(n), max_iter=iters, want_times=True)
+    dt = time.perf_counter() - t0
+    t = r["times"]
+    mv_s = t[2] / (r["num_iters"] + 1)
+    return {
+        "value": r["num_iters"] / dt, "unit": "CG iterations/s", "cores": 1, "kind": "port",
+        "sample": f"{r['num_iters']} of 200 FP64 CG iterations on the same 300^3 matrix "
+                  f"(oracle serial restatement of ref/ cg_solve, {dt:.1f} s)",
+        "spmv_GBs": spmv_bytes(n, nnz) / mv_s / 1e9,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=300, help="elements per side")
+    ap.add_argument("--iters", type=int, default=200, help="CG iterations per step")
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--cpu-iters", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tol-run", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(every GPU box here has one GPU); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import minife_amd as mfe
+
+    ctx = mfe.Context(0)
+    info = ctx.device_info()
+    log(f"device {info}")
+    N = args.n
+    ne = N ** 3
+
+    # ---- assembly (structure -> elements -> Dirichlet), timed by phase ------
+    mesh = mfe.simple_mesh_description([0, N, 0, N, 0, N])
+    A = mfe.CSRMatrix(ctx)
+    t0 = time.perf_counter()
+    assert mfe.generate_matrix_structure(mesh, A) == 0
+    ctx.synchronize()
+    t_struct = time.perf_counter() - t0
+    n, nnz = A.shape
+    b = mfe.Vector(n, ctx)
+    t0 = time.perf_counter()
+    mfe.assemble_FE_data(mesh, A, b)
+    ctx.synchronize()
+    t_asm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    mfe.impose_dirichlet(0.0, A, b, N + 1, N + 1, N + 1, mesh.bc_rows_0)
+    mfe.impose_dirichlet(1.0, A, b, N + 1, N + 1, N + 1, mesh.bc_rows_1)
+    ctx.synchronize()
+    t_bc = time.perf_counter() - t0
+    mfe.make_local_matrix(A)
+    log(f"assembled n={n} nnz={nnz}: struct {t_struct:.3f}s, FE {t_asm:.3f}s, BC {t_bc:.3f}s")
+
+    # ---- CG steps ----------------------------------------------------------
+    x = mfe.Vector(n, ctx)
+    for w in range(args.warmup):
+        x.fill(0.0)
+        r = mfe.cg_solve(A, b, x, max_iter=args.iters)
+        log(f"warmup {w}: iters {r.num_iters} normr {r.normr:.6e}")
+    ctx.synchronize()
+    iters_total = 0
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        x.fill(0.0)
+        r = mfe.cg_solve(A, b, x, max_iter=args.iters)
+        iters_total += r.num_iters
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    it_s = iters_total / dt
+    log(f"timed {args.steps} steps: {iters_total} iterations in {dt:.3f}s -> {it_s:.1f} it/s, "
+        f"final normr {r.normr:.6e}")
+
+    # ---- SpMV kernel rate (the CG's kernel and matvec_std's) ------------------
+    xs = mfe.Vector.from_numpy(np.linspace(0.0, 1.0, n), ctx)
+    ys = mfe.Vector(n, ctx)
+    mfe.time_matvec(A, xs, ys, variant=1, reps=3)
+    ms_cg = mfe.time_matvec(A, xs, ys, variant=1, reps=args.spmv_reps)
+    ms_std = mfe.time_matvec(A, xs, ys, variant=0, reps=args.spmv_reps)
+    byt = spmv_bytes(n, nnz)
+    gbs_cg = byt / (ms_cg * 1e-3) / 1e9
+    gbs_std = byt / (ms_std * 1e-3) / 1e9
+    log(f"spmv (cg kernel) {ms_cg:.4f} ms -> {gbs_cg:.1f} GB/s; matvec_std {ms_std:.4f} ms "
+        f"-> {gbs_std:.1f} GB/s")
+
+    # ---- time to ||r||/||b|| <= 1e-10 ----------------------------------------
+    tol_run = None
+    if not args.no_tol_run:
+        x.fill(0.0)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        r2 = mfe.cg_solve(A, b, x, max_iter=5000, tolerance=1e-10, relative=True)
+        tt = time.perf_counter() - t0
+        tol_run = {"fp64_s": tt, "fp64_iters": r2.num_iters, "fp64_normr": r2.normr}
+        log(f"time to 1e-10 (fp64): {tt:.3f}s, {r2.num_iters} iterations")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(A, b, n, nnz, args.cpu_iters)
+        if cpu:
+            log(f"cpu baseline {cpu['value']:.3f} it/s ({cpu['sample']})")
+
+    out = {
+        "metric": "CG iters/s + SpMV HBM GB/s at 300^3 FP64; time-to-1e-10 resid, mixed vs FP64",
+        "value": it_s,
+        "unit": "CG iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (miniFE hex8 box, unit conductivity, Q=1, u=1 on x=1)",
+        "config": {"workload": f"miniFE {N}x{N}x{N} hex8 box, FP64 CSR assembled on GPU, "
+                               f"unpreconditioned CG {args.iters} iterations",
+                   "nx": N, "ny": N, "nz": N, "nrows": n, "nnz": nnz,
+                   "cg_iters_per_step": args.iters},
+        "roofline": {"bound": "hbm", "achieved": gbs_cg, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs_cg / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "spmv_stream_kernel<false,true> (CG SpMV + p.Ap partials)",
+                     "bytes_per_launch": byt, "avg_ms": ms_cg},
+        "spmv_GBs": gbs_cg,
+        "spmv_std_GBs": gbs_std,
+        "assembly": {"struct_s": t_struct, "fe_s": t_asm, "dirichlet_s": t_bc,
+                     "elements_per_s": ne / t_asm},
+        "time_to_1e-10": tol_run,
+        "cpu_baseline": cpu,
+        "device": info["name"],
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,178 @@
+/*
+ * minife_hip.h -- C-ABI of libminife_hip.so, the MI355X (gfx950) implementation
+ * of miniFE's assemble-and-solve path.
+ *
+ * The reference (Mantevo miniFE, ref/) has no FFI: its operators are C++
+ * templates in namespace miniFE.  Each entry point below replaces one of them;
+ * the comment names the reference interface (file:line, relative to the
+ * reference root).  The C++ layer minife_amd/host/miniFE_hip.hpp keeps the
+ * reference names and signatures on top of these entry points.
+ *
+ * Conventions
+ *   - Plain C types only; every call returns an mfe_status (0 = MFE_OK).
+ *     mfe_last_error(ctx) holds a message for the last failure on ctx.
+ *   - One rank: local box == global box (the id->row map of
+ *     simple_mesh_description is the identity).  Boxes are int[6] =
+ *     {x0,x1,y0,y1,z0,z1} in elements, as ref/utils/Box.hpp.
+ *   - Ordinals are int32 and Scalar is double, as ref/src/Makefile.gnu.serial;
+ *     nnz must stay below 2^31 (the reference's LocalOrdinal is int).
+ *   - Handles own device memory.  Host pointers are borrowed for the call.
+ *     Calls on one ctx are ordered on its single HIP stream; a ctx is not
+ *     thread-safe.  No C++ exception crosses this boundary.
+ */
+#ifndef MINIFE_HIP_H
+#define MINIFE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum mfe_status {
+  MFE_OK = 0,
+  MFE_BREAKDOWN = 1,   /* cg_solve.hpp:186-200 numerical breakdown */
+  MFE_NOT_LOCAL = 2,   /* cg_solve.hpp:99-104 !A.has_local_indices */
+  MFE_OOM = 3,         /* generate_matrix_structure.hpp:80-147 allocation failure */
+  MFE_HIP_ERROR = 4,
+  MFE_BAD_ARG = 5,
+  MFE_UNSUPPORTED = 6
+} mfe_status;
+
+typedef struct mfe_ctx mfe_ctx;
+typedef struct mfe_csr mfe_csr;
+typedef struct mfe_vec mfe_vec;
+
+/* cg_solve flags */
+#define MFE_CG_REPLAY       0x1u /* sequential dots: reference's exact trajectory */
+#define MFE_CG_TOL_RELATIVE 0x2u /* stop on ||r|| <= tol*||b|| instead of ||r|| <= tol */
+#define MFE_CG_EAGER        0x4u /* launch kernels one by one instead of HIP graphs */
+
+/* Timer slots of cg_solve's times[5] (ref/utils/mytimer.hpp:36-43). */
+#define MFE_T_WAXPY 0
+#define MFE_T_DOT 1
+#define MFE_T_MATVEC 2
+#define MFE_T_MATVECDOT 3
+#define MFE_T_TOTAL 4
+
+/* ---- context ----------------------------------------------------------- */
+int mfe_ctx_create(int device, mfe_ctx** out);
+void mfe_ctx_destroy(mfe_ctx* ctx);
+const char* mfe_last_error(const mfe_ctx* ctx);
+int mfe_ctx_synchronize(mfe_ctx* ctx);
+/* Device name, compute-unit count and HBM bytes of ctx's device. */
+int mfe_device_info(mfe_ctx* ctx, char* name, int name_len, int* cu_count,
+                    int64_t* hbm_bytes);
+/* Stream-ordered timer on ctx's stream (hipEvent pair): start, then stop
+ * returns the elapsed milliseconds between the two marks. */
+int mfe_timer_start(mfe_ctx* ctx);
+int mfe_timer_stop(mfe_ctx* ctx, double* elapsed_ms);
+
+/* ---- CSRMatrix (ref/src/CSRMatrix.hpp:40-129) --------------------------- */
+/* CSRMatrix() constructor, CSRMatrix.hpp:45-54: empty matrix. */
+int mfe_csr_create(mfe_ctx* ctx, mfe_csr** out);
+void mfe_csr_destroy(mfe_csr* A);
+/* rows.size() and num_nonzeros(), CSRMatrix.hpp:84-87. */
+int mfe_csr_shape(const mfe_csr* A, int64_t* nrows, int64_t* nnz);
+/* Copy a host CSR in (rows may be NULL: rows[i] = i).  Replaces contents. */
+int mfe_csr_upload(mfe_ctx* ctx, mfe_csr* A, int64_t nrows, int64_t nnz, const int* rows,
+                   const int* row_offsets, const int* cols, const double* vals);
+/* Copy any subset of the arrays out (NULL pointers are skipped). */
+int mfe_csr_download(mfe_ctx* ctx, const mfe_csr* A, int* rows, int* row_offsets,
+                     int* cols, double* vals);
+
+/* ---- Vector (ref/src/Vector.hpp:36-61) ---------------------------------- */
+/* Vector(startIdx, local_sz), Vector.hpp:42-50: zero-filled. */
+int mfe_vec_create(mfe_ctx* ctx, int64_t n, mfe_vec** out);
+void mfe_vec_destroy(mfe_vec* v);
+int64_t mfe_vec_size(const mfe_vec* v);
+int mfe_vec_upload(mfe_ctx* ctx, mfe_vec* v, const double* host, int64_t n);
+int mfe_vec_download(mfe_ctx* ctx, const mfe_vec* v, double* host, int64_t n);
+int mfe_vec_fill(mfe_ctx* ctx, mfe_vec* v, double value);
+
+/* ---- assemble ------------------------------------------------------------ */
+/* generate_matrix_structure(mesh, A), generate_matrix_structure.hpp:47-149
+ * (+ MatrixInitOp<CSRMatrix> MatrixInitOp.hpp:127-156): rows, row_offsets,
+ * ascending 27-stencil cols, zero coefficients.  Bit-exact. */
+int mfe_generate_matrix_structure(mfe_ctx* ctx, const int global_box[6],
+                                  const int local_box[6], mfe_csr* A);
+
+/* perform_element_loop(mesh, elem_box, A, b, params),
+ * perform_element_loop.hpp:40-104: hex8 diffusion matrix and source vector per
+ * element (Hex8.hpp:259-410), summed into A and b.  Each entry receives its
+ * elements' contributions in ascending element id, from 0.0, so the result
+ * is bit-identical to the reference.  A must hold the structure of
+ * mfe_generate_matrix_structure; values and b are overwritten.  kappa
+ * (nullable, one double per element) is the per-element conductivity k of
+ * Hex8.hpp:273 (reference: 1.0). */
+int mfe_perform_element_loop(mfe_ctx* ctx, const int global_box[6], const int elem_box[6],
+                             mfe_csr* A, mfe_vec* b, const mfe_vec* kappa);
+/* assemble_FE_data(mesh, A, b, params), assemble_FE_data.hpp:38-73. */
+int mfe_assemble_FE_data(mfe_ctx* ctx, const int global_box[6], const int local_box[6],
+                         mfe_csr* A, mfe_vec* b, const mfe_vec* kappa);
+
+/* impose_dirichlet(value, A, b, gnx, gny, gnz, bc_rows),
+ * SparseMatrix_functions.hpp:431-480: bc_rows is the reference's std::set as a
+ * sorted host array.  BC rows become identity rows with b = value; other rows
+ * drop their BC columns and subtract their (column-ordered) sum * value. */
+int mfe_impose_dirichlet(mfe_ctx* ctx, double prescribed_value, mfe_csr* A, mfe_vec* b,
+                         int global_nx, int global_ny, int global_nz, const int* bc_rows,
+                         int64_t num_bc_rows);
+
+/* make_local_matrix(A), make_local_matrix.hpp:42-442 (one rank: num_cols =
+ * nrows, has_local_indices = true). */
+int mfe_make_local_matrix(mfe_ctx* ctx, mfe_csr* A);
+
+/* ---- solve --------------------------------------------------------------- */
+/* matvec_std<CSRMatrix>::operator()(A, x, y), SparseMatrix_functions.hpp:545-578.
+ * Each row's products are summed in stored column order: bit-identical. */
+int mfe_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y);
+/* matvec_and_dot(A, x, y), SparseMatrix_functions.hpp:491-536: y = A x and
+ * returns x.y (two-level tree reduction). */
+int mfe_matvec_and_dot(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y,
+                       double* result);
+/* dot(x, y), Vector_functions.hpp:194-226, as a deterministic two-level
+ * (wave64 shuffle + LDS) tree reduction. */
+int mfe_dot(mfe_ctx* ctx, const mfe_vec* x, const mfe_vec* y, double* result);
+/* dot(x, y) summed strictly left to right, as the reference: bit-identical. */
+int mfe_dot_replay(mfe_ctx* ctx, const mfe_vec* x, const mfe_vec* y, double* result);
+/* waxpby(alpha, x, beta, y, w), Vector_functions.hpp:127-150. */
+int mfe_waxpby(mfe_ctx* ctx, double alpha, const mfe_vec* x, double beta, const mfe_vec* y,
+               mfe_vec* w);
+
+/* cg_solve(A, b, x, matvec_std, max_iter, tolerance, num_iters, normr, times),
+ * cg_solve.hpp:69-229: unpreconditioned CG from the initial x.  Iteration k
+ * runs while k <= max_iter and the residual norm computed at the start of the
+ * previous iteration exceeds tolerance (the reference's loop test), so
+ * num_iters and normr mean what they mean there.  Returns MFE_OK or
+ * MFE_BREAKDOWN (num_iters = last completed k).  times (nullable) receives
+ * seconds in the MFE_T_* slots; history (nullable) receives 4 doubles per
+ * iteration k (entry k-1): {rtrans, p_ap_dot, alpha, beta}.
+ * flags: MFE_CG_REPLAY reproduces the reference's sequential dot products and
+ * hence its scalar trajectory bit for bit; the default fast mode uses tree
+ * reductions (deterministic, not bit-identical to the serial sum). */
+int mfe_cg_solve(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x, int max_iter,
+                 double tolerance, unsigned flags, int* num_iters, double* normr,
+                 double* times, double* history);
+
+/* Kernel timing mode, the counterpart of time_kernels (time_kernels.hpp:42-134):
+ * launches `reps` back-to-back SpMV kernels y = A x on ctx's stream between two
+ * hipEvents and returns the mean kernel time in ms.  variant 0 = matvec_std's
+ * kernel (row sums start at 0*y[row]); variant 1 = the kernel inside the fast
+ * cg_solve (sums start at +0.0, plus workgroup partials of x.y, y.y, x.x). */
+int mfe_time_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y, int variant,
+                    int reps, double* avg_ms);
+
+/* ---- verification (host) -------------------------------------------------- */
+/* soln(x, y, z, max_p, max_q), analytic_soln.hpp:80-112. */
+double mfe_soln(double x, double y, double z, int max_p, int max_q);
+/* verify_solution(mesh, x, tol, whole), verify_solution.hpp:55-174: returns 0
+ * when the max abs error over the checked nodes is <= tolerance, else 1;
+ * *max_err (nullable) receives that error. */
+int mfe_verify_solution(mfe_ctx* ctx, const int global_box[6], const mfe_vec* x,
+                        double tolerance, int verify_whole_domain, double* max_err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,462 @@
+"""minife_amd -- miniFE's assemble-and-solve path on MI355X (gfx950).
+
+Python mirror of the reference ref/ operator surface over the C-ABI of
+``libminife_hip.so`` (include/minife_hip.h).  Names, argument meaning and error
+behaviour follow the reference:
+
+  simple_mesh_description(global_box, local_box)   simple_mesh_description.hpp:38-244
+  generate_matrix_structure(mesh, A)                generate_matrix_structure.hpp:47-149
+  assemble_FE_data(mesh, A, b, params)              assemble_FE_data.hpp:38-73
+  perform_element_loop(mesh, box, A, b, params)     perform_element_loop.hpp:40-104
+  impose_dirichlet(value, A, b, gnx, gny, gnz, bc)  SparseMatrix_functions.hpp:431-480
+  make_local_matrix(A)                              make_local_matrix.hpp:42-442
+  cg_solve(A, b, x, max_iter, tol)                  cg_solve.hpp:69-229
+  matvec / matvec_and_dot / dot / waxpby            SparseMatrix_functions.hpp:491-578,
+                                                    Vector_functions.hpp:127-226
+  verify_solution(mesh, x, tol, whole)              verify_solution.hpp:55-174
+
+Matrices and vectors live in HBM (device handles); ``to_host`` copies out.
+The HIP library is required: importing works without it, but the first call
+raises if ``libminife_hip.so`` is missing or no GPU is visible.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libminife_hip.so")
+
+MFE_OK, MFE_BREAKDOWN, MFE_NOT_LOCAL, MFE_OOM, MFE_HIP_ERROR, MFE_BAD_ARG, MFE_UNSUPPORTED = range(7)
+STATUS_NAMES = ["OK", "BREAKDOWN", "NOT_LOCAL", "OOM", "HIP_ERROR", "BAD_ARG", "UNSUPPORTED"]
+CG_REPLAY, CG_TOL_RELATIVE, CG_EAGER = 0x1, 0x2, 0x4
+T_WAXPY, T_DOT, T_MATVEC, T_MATVECDOT, T_TOTAL = range(5)
+
+# Every extern "C" symbol declared in include/minife_hip.h.
+EXPORTS = [
+    "mfe_ctx_create", "mfe_ctx_destroy", "mfe_last_error", "mfe_ctx_synchronize",
+    "mfe_device_info", "mfe_timer_start", "mfe_timer_stop",
+    "mfe_csr_create", "mfe_csr_destroy", "mfe_csr_shape", "mfe_csr_upload", "mfe_csr_download",
+    "mfe_vec_create", "mfe_vec_destroy", "mfe_vec_size", "mfe_vec_upload", "mfe_vec_download",
+    "mfe_vec_fill",
+    "mfe_generate_matrix_structure", "mfe_perform_element_loop", "mfe_assemble_FE_data",
+    "mfe_impose_dirichlet", "mfe_make_local_matrix",
+    "mfe_matvec", "mfe_matvec_and_dot", "mfe_dot", "mfe_dot_replay", "mfe_waxpby",
+    "mfe_cg_solve", "mfe_time_matvec", "mfe_soln", "mfe_verify_solution",
+]
+
+
+class MfeError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        name = STATUS_NAMES[status] if 0 <= status < len(STATUS_NAMES) else str(status)
+        super().__init__(f"{name}: {msg}")
+        self.status = status
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_int6 = ctypes.c_int * 6
+
+
+def lib() -> ctypes.CDLL:
+    """Load libminife_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m minife_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "mfe_ctx_create": ([ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+        "mfe_ctx_destroy": ([_vp], None),
+        "mfe_last_error": ([_vp], ctypes.c_char_p),
+        "mfe_ctx_synchronize": ([_vp], ctypes.c_int),
+        "mfe_device_info": ([_vp, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                             ctypes.POINTER(_i64)], ctypes.c_int),
+        "mfe_timer_start": ([_vp], ctypes.c_int),
+        "mfe_timer_stop": ([_vp, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "mfe_csr_create": ([_vp, ctypes.POINTER(_vp)], ctypes.c_int),
+        "mfe_csr_destroy": ([_vp], None),
+        "mfe_csr_shape": ([_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64)], ctypes.c_int),
+        "mfe_csr_upload": ([_vp, _vp, _i64, _i64, _vp, _vp, _vp, _vp], ctypes.c_int),
+        "mfe_csr_download": ([_vp, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+        "mfe_vec_create": ([_vp, _i64, ctypes.POINTER(_vp)], ctypes.c_int),
+        "mfe_vec_destroy": ([_vp], None),
+        "mfe_vec_size": ([_vp], _i64),
+        "mfe_vec_upload": ([_vp, _vp, _vp, _i64], ctypes.c_int),
+        "mfe_vec_download": ([_vp, _vp, _vp, _i64], ctypes.c_int),
+        "mfe_vec_fill": ([_vp, _vp, ctypes.c_double], ctypes.c_int),
+        "mfe_generate_matrix_structure": ([_vp, _int6, _int6, _vp], ctypes.c_int),
+        "mfe_perform_element_loop": ([_vp, _int6, _int6, _vp, _vp, _vp], ctypes.c_int),
+        "mfe_assemble_FE_data": ([_vp, _int6, _int6, _vp, _vp, _vp], ctypes.c_int),
+        "mfe_impose_dirichlet": ([_vp, ctypes.c_double, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, _vp, _i64], ctypes.c_int),
+        "mfe_make_local_matrix": ([_vp, _vp], ctypes.c_int),
+        "mfe_matvec": ([_vp, _vp, _vp, _vp], ctypes.c_int),
+        "mfe_matvec_and_dot": ([_vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "mfe_dot": ([_vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "mfe_dot_replay": ([_vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "mfe_waxpby": ([_vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vp], ctypes.c_int),
+        "mfe_cg_solve": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_uint,
+                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), _vp, _vp],
+                         ctypes.c_int),
+        "mfe_time_matvec": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                             ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "mfe_soln": ([ctypes.c_double] * 3 + [ctypes.c_int, ctypes.c_int], ctypes.c_double),
+        "mfe_verify_solution": ([_vp, _int6, _vp, ctypes.c_double, ctypes.c_int,
+                                 ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(_vp)
+
+
+class Context:
+    """One device and one HIP stream (mfe_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        st = lib().mfe_ctx_create(int(device), ctypes.byref(h))
+        if st != MFE_OK:
+            raise MfeError(st, f"mfe_ctx_create(device={device}) failed (no GPU visible?)")
+        self.h = h
+        self.device = device
+
+    def check(self, st: int, what: str = ""):
+        if st != MFE_OK:
+            raise MfeError(st, f"{what}: {lib().mfe_last_error(self.h).decode(errors='replace')}")
+
+    def synchronize(self):
+        self.check(lib().mfe_ctx_synchronize(self.h), "synchronize")
+
+    def device_info(self) -> dict:
+        buf = ctypes.create_string_buffer(256)
+        cu = ctypes.c_int(0)
+        hbm = _i64(0)
+        self.check(lib().mfe_device_info(self.h, buf, 256, ctypes.byref(cu), ctypes.byref(hbm)),
+                   "device_info")
+        return dict(name=buf.value.decode(), cu_count=cu.value, hbm_bytes=hbm.value)
+
+    def timer_start(self):
+        self.check(lib().mfe_timer_start(self.h), "timer_start")
+
+    def timer_stop(self) -> float:
+        ms = ctypes.c_double(0)
+        self.check(lib().mfe_timer_stop(self.h, ctypes.byref(ms)), "timer_stop")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mfe_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx: Context | None = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(int(os.environ.get("MFE_DEVICE", "0")))
+    return _default_ctx
+
+
+class CSRMatrix:
+    """Device CSR with the fields of CSRMatrix.hpp:40-129 (int32 ordinals, float64)."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        h = _vp()
+        self.ctx.check(lib().mfe_csr_create(self.ctx.h, ctypes.byref(h)), "csr_create")
+        self.h = h
+
+    @property
+    def shape(self):
+        n, z = _i64(), _i64()
+        lib().mfe_csr_shape(self.h, ctypes.byref(n), ctypes.byref(z))
+        return n.value, z.value
+
+    @property
+    def nrows(self) -> int:
+        return self.shape[0]
+
+    def num_nonzeros(self) -> int:
+        return self.shape[1]
+
+    @classmethod
+    def from_host(cls, row_offsets, cols, vals, rows=None, ctx: Context | None = None):
+        A = cls(ctx)
+        ro = np.ascontiguousarray(row_offsets, dtype=np.int32)
+        c = np.ascontiguousarray(cols, dtype=np.int32)
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        r = None if rows is None else np.ascontiguousarray(rows, dtype=np.int32)
+        A.ctx.check(lib().mfe_csr_upload(A.ctx.h, A.h, ro.shape[0] - 1, c.shape[0], _ptr(r),
+                                         _ptr(ro), _ptr(c), _ptr(v)), "csr_upload")
+        return A
+
+    def to_host(self, rows=True, row_offsets=True, cols=True, vals=True) -> dict:
+        n, nnz = self.shape
+        out = {}
+        if rows:
+            out["rows"] = np.empty(n, np.int32)
+        if row_offsets:
+            out["row_offsets"] = np.empty(n + 1, np.int32)
+        if cols:
+            out["cols"] = np.empty(nnz, np.int32)
+        if vals:
+            out["vals"] = np.empty(nnz, np.float64)
+        self.ctx.check(lib().mfe_csr_download(self.ctx.h, self.h, _ptr(out.get("rows")),
+                                              _ptr(out.get("row_offsets")), _ptr(out.get("cols")),
+                                              _ptr(out.get("vals"))), "csr_download")
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mfe_csr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Vector:
+    """Device vector of float64 (Vector.hpp:36-61); zero-filled on creation."""
+
+    def __init__(self, n: int, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        h = _vp()
+        self.ctx.check(lib().mfe_vec_create(self.ctx.h, int(n), ctypes.byref(h)), "vec_create")
+        self.h = h
+        self.n = int(n)
+
+    @classmethod
+    def from_numpy(cls, a, ctx: Context | None = None):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        v = cls(a.shape[0], ctx)
+        v.ctx.check(lib().mfe_vec_upload(v.ctx.h, v.h, _ptr(a), a.shape[0]), "vec_upload")
+        return v
+
+    def to_numpy(self) -> np.ndarray:
+        out = np.empty(self.n, np.float64)
+        self.ctx.check(lib().mfe_vec_download(self.ctx.h, self.h, _ptr(out), self.n), "vec_download")
+        return out
+
+    def fill(self, value: float):
+        self.ctx.check(lib().mfe_vec_fill(self.ctx.h, self.h, float(value)), "vec_fill")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mfe_vec_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _box6(box) -> "ctypes.Array":
+    b = np.asarray(box, dtype=np.int64).reshape(-1)
+    if b.shape[0] != 6:
+        raise ValueError("a Box is 6 ints {x0,x1,y0,y1,z0,z1}")
+    return _int6(*[int(v) for v in b])
+
+
+@dataclass
+class Parameters:
+    """Runtime parameters of ref/utils/Parameters.hpp:40-90 used on this path."""
+    nx: int = 10
+    ny: int = -1
+    nz: int = -1
+    verify_solution: int = 0
+    name: str = ""
+    extra: dict = field(default_factory=dict)
+
+    def __post_init__(self):
+        if self.ny < 0:
+            self.ny = self.nx
+        if self.nz < 0:
+            self.nz = self.ny
+
+
+class simple_mesh_description:
+    """simple_mesh_description.hpp:38-244 for one rank: boxes and BC row sets.
+
+    bc_rows_0: nodes on the faces x=0, y=0, z=0, y=1, z=1 (prescribed 0);
+    bc_rows_1: nodes on the face x=1 (prescribed 1).  Sorted int32 arrays
+    (the reference's std::set); one rank so node id == row.
+    """
+
+    def __init__(self, global_box, local_box=None):
+        self.global_box = np.asarray(global_box, dtype=np.int64).reshape(6)
+        self.local_box = self.global_box.copy() if local_box is None else \
+            np.asarray(local_box, dtype=np.int64).reshape(6)
+        gb = self.global_box
+        if gb[0] != 0 or gb[2] != 0 or gb[4] != 0 or min(gb[1], gb[3], gb[5]) < 1:
+            raise ValueError("global box must be {0,nx,0,ny,0,nz} with nx,ny,nz >= 1")
+        if not np.array_equal(self.local_box, self.global_box):
+            raise NotImplementedError("one rank: local box must equal the global box")
+        gx, gy, gz = int(gb[1]) + 1, int(gb[3]) + 1, int(gb[5]) + 1
+        iz, iy, ix = np.meshgrid(np.arange(gz), np.arange(gy), np.arange(gx), indexing="ij")
+        ids = (ix + gx * iy + gx * gy * iz).reshape(-1)
+        ix, iy, iz = ix.reshape(-1), iy.reshape(-1), iz.reshape(-1)
+        in0 = (ix == 0) | (iy == 0) | (iz == 0) | (iy == gy - 1) | (iz == gz - 1)
+        in1 = ix == gx - 1
+        self.bc_rows_0 = ids[in0].astype(np.int32)
+        self.bc_rows_1 = ids[in1].astype(np.int32)
+
+    @property
+    def nodes(self):
+        gb = self.global_box
+        return int(gb[1]) + 1, int(gb[3]) + 1, int(gb[5]) + 1
+
+
+def generate_matrix_structure(mesh: simple_mesh_description, A: CSRMatrix) -> int:
+    """Returns 0 on success, 1 on allocation failure (generate_matrix_structure.hpp:140-148)."""
+    st = lib().mfe_generate_matrix_structure(A.ctx.h, _box6(mesh.global_box),
+                                             _box6(mesh.local_box), A.h)
+    if st == MFE_OOM:
+        return 1
+    A.ctx.check(st, "generate_matrix_structure")
+    return 0
+
+
+def perform_element_loop(mesh, elem_box, A: CSRMatrix, b: Vector, params=None, kappa=None):
+    st = lib().mfe_perform_element_loop(A.ctx.h, _box6(mesh.global_box), _box6(elem_box), A.h,
+                                        b.h, kappa.h if kappa is not None else None)
+    A.ctx.check(st, "perform_element_loop")
+
+
+def assemble_FE_data(mesh, A: CSRMatrix, b: Vector, params=None, kappa=None):
+    st = lib().mfe_assemble_FE_data(A.ctx.h, _box6(mesh.global_box), _box6(mesh.local_box), A.h,
+                                    b.h, kappa.h if kappa is not None else None)
+    A.ctx.check(st, "assemble_FE_data")
+
+
+def impose_dirichlet(prescribed_value: float, A: CSRMatrix, b: Vector, global_nx: int,
+                     global_ny: int, global_nz: int, bc_rows) -> None:
+    bc = np.unique(np.asarray(bc_rows, dtype=np.int32))  # std::set semantics
+    st = lib().mfe_impose_dirichlet(A.ctx.h, float(prescribed_value), A.h, b.h, int(global_nx),
+                                    int(global_ny), int(global_nz), _ptr(bc), bc.shape[0])
+    A.ctx.check(st, "impose_dirichlet")
+
+
+def make_local_matrix(A: CSRMatrix) -> None:
+    A.ctx.check(lib().mfe_make_local_matrix(A.ctx.h, A.h), "make_local_matrix")
+
+
+def matvec(A: CSRMatrix, x: Vector, y: Vector) -> None:
+    A.ctx.check(lib().mfe_matvec(A.ctx.h, A.h, x.h, y.h), "matvec")
+
+
+def matvec_and_dot(A: CSRMatrix, x: Vector, y: Vector) -> float:
+    r = ctypes.c_double(0)
+    A.ctx.check(lib().mfe_matvec_and_dot(A.ctx.h, A.h, x.h, y.h, ctypes.byref(r)), "matvec_and_dot")
+    return r.value
+
+
+def dot(x: Vector, y: Vector, replay: bool = False) -> float:
+    r = ctypes.c_double(0)
+    f = lib().mfe_dot_replay if replay else lib().mfe_dot
+    x.ctx.check(f(x.ctx.h, x.h, y.h, ctypes.byref(r)), "dot")
+    return r.value
+
+
+def waxpby(alpha: float, x: Vector, beta: float, y: Vector, w: Vector) -> None:
+    x.ctx.check(lib().mfe_waxpby(x.ctx.h, float(alpha), x.h, float(beta), y.h, w.h), "waxpby")
+
+
+@dataclass
+class CGResult:
+    num_iters: int
+    normr: float
+    status: int
+    times: np.ndarray
+    history: np.ndarray | None  # [k, 4] = rtrans, p_ap_dot, alpha, beta
+
+    @property
+    def breakdown(self) -> bool:
+        return self.status == MFE_BREAKDOWN
+
+
+def cg_solve(A: CSRMatrix, b: Vector, x: Vector, max_iter: int = 200,
+             tolerance: float = float(np.finfo(np.float64).eps), *, replay: bool = False,
+             relative: bool = False, eager: bool = False, history: bool = False) -> CGResult:
+    """cg_solve.hpp:69-229.  Breakdown is reported in the result (status), as
+    the reference returns early rather than failing."""
+    flags = (CG_REPLAY if replay else 0) | (CG_TOL_RELATIVE if relative else 0) | \
+        (CG_EAGER if eager else 0)
+    ni = ctypes.c_int(0)
+    nr = ctypes.c_double(0)
+    times = np.zeros(5, np.float64)
+    hist = np.zeros((max(int(max_iter), 1) + 1, 4), np.float64) if history else None
+    st = lib().mfe_cg_solve(A.ctx.h, A.h, b.h, x.h, int(max_iter), float(tolerance), flags,
+                            ctypes.byref(ni), ctypes.byref(nr), _ptr(times), _ptr(hist))
+    if st not in (MFE_OK, MFE_BREAKDOWN):
+        A.ctx.check(st, "cg_solve")
+    if hist is not None:
+        hist = hist[: ni.value + (1 if st == MFE_BREAKDOWN else 0)]
+    return CGResult(ni.value, nr.value, st, times, hist)
+
+
+def time_matvec(A: CSRMatrix, x: Vector, y: Vector, variant: int = 1, reps: int = 20) -> float:
+    """Mean SpMV kernel time in ms over `reps` back-to-back launches (HIP events
+    on the context's stream); variant 1 is the kernel inside the fast cg_solve."""
+    ms = ctypes.c_double(0)
+    A.ctx.check(lib().mfe_time_matvec(A.ctx.h, A.h, x.h, y.h, int(variant), int(reps),
+                                      ctypes.byref(ms)), "time_matvec")
+    return ms.value
+
+
+def soln(x: float, y: float, z: float, max_p: int = 300, max_q: int = 300) -> float:
+    return lib().mfe_soln(float(x), float(y), float(z), int(max_p), int(max_q))
+
+
+def verify_solution(mesh: simple_mesh_description, x: Vector, tolerance: float = 0.06,
+                    verify_whole_domain: bool = False):
+    """Returns (rc, max_abs_err); rc = 0 when within tolerance."""
+    e = ctypes.c_double(0)
+    rc = lib().mfe_verify_solution(x.ctx.h, _box6(mesh.global_box), x.h, float(tolerance),
+                                   int(bool(verify_whole_domain)), ctypes.byref(e))
+    if rc not in (0, 1):
+        x.ctx.check(rc, "verify_solution")
+    return rc, e.value
+
+
+def build_problem(nx: int, ny: int | None = None, nz: int | None = None, ctx: Context | None = None,
+                  kappa: Vector | None = None):
+    """Structure -> assembly -> Dirichlet (u=0 faces, then u=1 face) -> make_local_matrix,
+    the order of driver.hpp:159-230.  Returns (mesh, A, b)."""
+    ny = nx if ny is None else ny
+    nz = ny if nz is None else nz
+    mesh = simple_mesh_description([0, nx, 0, ny, 0, nz])
+    A = CSRMatrix(ctx)
+    if generate_matrix_structure(mesh, A) != 0:
+        raise MfeError(MFE_OOM, "generate_matrix_structure ran out of device memory")
+    b = Vector(A.nrows, A.ctx)
+    assemble_FE_data(mesh, A, b, kappa=kappa)
+    impose_dirichlet(0.0, A, b, nx + 1, ny + 1, nz + 1, mesh.bc_rows_0)
+    impose_dirichlet(1.0, A, b, nx + 1, ny + 1, nz + 1, mesh.bc_rows_1)
+    make_local_matrix(A)
+    return mesh, A, b

@@ -1,0 +1,399 @@
+// Context, handle, timer and verification entry points of libminife_hip.so.
+#include <cmath>
+#include <cstring>
+
+#include "mfe_common.hpp"
+
+namespace mfe {
+
+int fail(mfe_ctx* ctx, int status, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return status;
+}
+
+int hip_fail(mfe_ctx* ctx, hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  return fail(ctx, e == hipErrorOutOfMemory ? MFE_OOM : MFE_HIP_ERROR, m);
+}
+
+// Largest non-zero count of any kSpmvRows-row tile: ro[min(r0+R,n)] - ro[r0].
+__global__ void tile_nnz_max_kernel(int64_t n, const int* __restrict__ ro, int64_t ntiles,
+                                    unsigned long long* out) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  int64_t r0 = t * kSpmvRows;
+  int64_t r1 = r0 + kSpmvRows < n ? r0 + kSpmvRows : n;
+  long long c = (long long)ro[r1] - (long long)ro[r0];
+  atomicMax(out, (unsigned long long)(c < 0 ? 0 : c));
+}
+
+int plan_spmv(mfe_ctx* ctx, mfe_csr* A) {
+  if (A->nrows == 0) {
+    A->max_block_nnz = 0;
+    return MFE_OK;
+  }
+  int64_t ntiles = cdiv(A->nrows, kSpmvRows);
+  unsigned long long* d = reinterpret_cast<unsigned long long*>(ctx->d_scalars);
+  MFE_CHECK(ctx, hipMemsetAsync(d, 0, sizeof(*d), ctx->stream));
+  tile_nnz_max_kernel<<<(unsigned)cdiv(ntiles, kBlock), kBlock, 0, ctx->stream>>>(
+      A->nrows, A->row_offsets, ntiles, d);
+  MFE_LAUNCHED(ctx, "tile_nnz_max_kernel");
+  MFE_CHECK(ctx, hipMemcpyAsync(ctx->h_scalars, d, sizeof(*d), hipMemcpyDeviceToHost,
+                                ctx->stream));
+  MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  unsigned long long v;
+  std::memcpy(&v, ctx->h_scalars, sizeof(v));
+  A->max_block_nnz = (int64_t)v;
+  return MFE_OK;
+}
+
+}  // namespace mfe
+
+using namespace mfe;
+
+extern "C" {
+
+int mfe_ctx_create(int device, mfe_ctx** out) {
+  if (!out) return MFE_BAD_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return MFE_HIP_ERROR;
+  if (device < 0 || device >= ndev) return MFE_BAD_ARG;
+  mfe_ctx* c = new mfe_ctx();
+  c->device = device;
+  auto bail = [&](int st) {
+    mfe_ctx_destroy(c);
+    return st;
+  };
+  if (hipSetDevice(device) != hipSuccess) return bail(MFE_HIP_ERROR);
+  // Default-priority, non-blocking stream (no priority variants are used).
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(MFE_HIP_ERROR);
+  if (hipEventCreate(&c->t_start) != hipSuccess || hipEventCreate(&c->t_stop) != hipSuccess)
+    return bail(MFE_HIP_ERROR);
+  if (hipMalloc(&c->d_scalars, 64 * sizeof(double)) != hipSuccess) return bail(MFE_OOM);
+  if (hipHostMalloc(&c->h_scalars, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    return bail(MFE_OOM);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+  *out = c;
+  return MFE_OK;
+}
+
+void mfe_cg_workspace_free(mfe_ctx* ctx);  // mfe_solve.hip
+
+void mfe_ctx_destroy(mfe_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  mfe_cg_workspace_free(c);
+  if (c->d_partials) hipFree(c->d_partials);
+  if (c->d_scalars) hipFree(c->d_scalars);
+  if (c->h_scalars) hipHostFree(c->h_scalars);
+  if (c->t_start) hipEventDestroy(c->t_start);
+  if (c->t_stop) hipEventDestroy(c->t_stop);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* mfe_last_error(const mfe_ctx* ctx) { return ctx ? ctx->err.c_str() : "null ctx"; }
+
+int mfe_ctx_synchronize(mfe_ctx* ctx) {
+  if (!ctx) return MFE_BAD_ARG;
+  MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return MFE_OK;
+}
+
+int mfe_device_info(mfe_ctx* ctx, char* name, int name_len, int* cu_count, int64_t* hbm) {
+  if (!ctx) return MFE_BAD_ARG;
+  hipDeviceProp_t prop;
+  MFE_CHECK(ctx, hipGetDeviceProperties(&prop, ctx->device));
+  if (name && name_len > 0) {
+    std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  if (hbm) *hbm = (int64_t)prop.totalGlobalMem;
+  return MFE_OK;
+}
+
+int mfe_timer_start(mfe_ctx* ctx) {
+  if (!ctx) return MFE_BAD_ARG;
+  MFE_CHECK(ctx, hipEventRecord(ctx->t_start, ctx->stream));
+  return MFE_OK;
+}
+
+int mfe_timer_stop(mfe_ctx* ctx, double* ms) {
+  if (!ctx || !ms) return MFE_BAD_ARG;
+  MFE_CHECK(ctx, hipEventRecord(ctx->t_stop, ctx->stream));
+  MFE_CHECK(ctx, hipEventSynchronize(ctx->t_stop));
+  float f = 0;
+  MFE_CHECK(ctx, hipEventElapsedTime(&f, ctx->t_start, ctx->t_stop));
+  *ms = f;
+  return MFE_OK;
+}
+
+// ---- CSRMatrix ------------------------------------------------------------
+
+static void csr_release(mfe_csr* A) {
+  hipFree(A->rows);
+  hipFree(A->row_offsets);
+  hipFree(A->cols);
+  hipFree(A->vals);
+  A->rows = A->row_offsets = A->cols = nullptr;
+  A->vals = nullptr;
+  A->nrows = A->nnz = 0;
+  A->max_block_nnz = -1;
+  A->has_local_indices = false;
+  A->num_cols = 0;
+}
+
+int mfe_csr_create(mfe_ctx* ctx, mfe_csr** out) {
+  if (!ctx || !out) return MFE_BAD_ARG;
+  *out = new mfe_csr();
+  (*out)->ctx = ctx;
+  return MFE_OK;
+}
+
+void mfe_csr_destroy(mfe_csr* A) {
+  if (!A) return;
+  hipSetDevice(A->ctx->device);
+  hipStreamSynchronize(A->ctx->stream);
+  csr_release(A);
+  delete A;
+}
+
+int mfe_csr_shape(const mfe_csr* A, int64_t* nrows, int64_t* nnz) {
+  if (!A) return MFE_BAD_ARG;
+  if (nrows) *nrows = A->nrows;
+  if (nnz) *nnz = A->nnz;
+  return MFE_OK;
+}
+
+}  // extern "C"
+
+namespace mfe {
+// (Re)allocate A's arrays for nrows/nnz; contents undefined.
+int csr_alloc(mfe_ctx* ctx, mfe_csr* A, int64_t nrows, int64_t nnz) {
+  MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  csr_release(A);
+  if (nrows < 0 || nnz < 0 || nnz > INT32_MAX || nrows >= INT32_MAX)
+    return fail(ctx, MFE_UNSUPPORTED, "matrix exceeds int32 ordinals");
+  MFE_CHECK(ctx, hipMalloc(&A->rows, sizeof(int) * (size_t)(nrows > 0 ? nrows : 1)));
+  MFE_CHECK(ctx, hipMalloc(&A->row_offsets, sizeof(int) * (size_t)(nrows + 1)));
+  MFE_CHECK(ctx, hipMalloc(&A->cols, sizeof(int) * (size_t)(nnz > 0 ? nnz : 1)));
+  MFE_CHECK(ctx, hipMalloc(&A->vals, sizeof(double) * (size_t)(nnz > 0 ? nnz : 1)));
+  A->nrows = nrows;
+  A->nnz = nnz;
+  return MFE_OK;
+}
+
+__global__ void iota_kernel(int64_t n, int* out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = (int)i;
+}
+}  // namespace mfe
+
+extern "C" {
+
+int mfe_csr_upload(mfe_ctx* ctx, mfe_csr* A, int64_t nrows, int64_t nnz, const int* rows,
+                   const int* row_offsets, const int* cols, const double* vals) {
+  if (!ctx || !A || !row_offsets || (nnz > 0 && (!cols || !vals))) return MFE_BAD_ARG;
+  MFE_CHECK(ctx, hipSetDevice(ctx->device));
+  if (row_offsets[0] != 0 || row_offsets[nrows] != nnz)
+    return fail(ctx, MFE_BAD_ARG, "row_offsets must start at 0 and end at nnz");
+  for (int64_t i = 0; i < nrows; ++i)
+    if (row_offsets[i + 1] < row_offsets[i])
+      return fail(ctx, MFE_BAD_ARG, "row_offsets must be non-decreasing");
+  for (int64_t j = 0; j < nnz; ++j)
+    if (cols[j] < 0 || cols[j] >= nrows)
+      return fail(ctx, MFE_BAD_ARG, "column index out of range (one rank: cols < nrows)");
+  int st = csr_alloc(ctx, A, nrows, nnz);
+  if (st) return st;
+  hipStream_t s = ctx->stream;
+  if (rows) {
+    MFE_CHECK(ctx, hipMemcpyAsync(A->rows, rows, sizeof(int) * nrows, hipMemcpyHostToDevice, s));
+  } else if (nrows > 0) {
+    iota_kernel<<<(unsigned)cdiv(nrows, kBlock), kBlock, 0, s>>>(nrows, A->rows);
+    MFE_LAUNCHED(ctx, "iota_kernel");
+  }
+  MFE_CHECK(ctx, hipMemcpyAsync(A->row_offsets, row_offsets, sizeof(int) * (nrows + 1),
+                                hipMemcpyHostToDevice, s));
+  if (nnz > 0) {
+    MFE_CHECK(ctx, hipMemcpyAsync(A->cols, cols, sizeof(int) * nnz, hipMemcpyHostToDevice, s));
+    MFE_CHECK(ctx,
+              hipMemcpyAsync(A->vals, vals, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+  }
+  A->has_local_indices = false;
+  A->num_cols = nrows;
+  return plan_spmv(ctx, A);
+}
+
+int mfe_csr_download(mfe_ctx* ctx, const mfe_csr* A, int* rows, int* row_offsets, int* cols,
+                     double* vals) {
+  if (!ctx || !A) return MFE_BAD_ARG;
+  MFE_CHECK(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  if (rows && A->nrows)
+    MFE_CHECK(ctx, hipMemcpyAsync(rows, A->rows, sizeof(int) * A->nrows, hipMemcpyDeviceToHost, s));
+  if (row_offsets && A->row_offsets)
+    MFE_CHECK(ctx, hipMemcpyAsync(row_offsets, A->row_offsets, sizeof(int) * (A->nrows + 1),
+                                  hipMemcpyDeviceToHost, s));
+  if (cols && A->nnz)
+    MFE_CHECK(ctx, hipMemcpyAsync(cols, A->cols, sizeof(int) * A->nnz, hipMemcpyDeviceToHost, s));
+  if (vals && A->nnz)
+    MFE_CHECK(ctx,
+              hipMemcpyAsync(vals, A->vals, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, s));
+  MFE_CHECK(ctx, hipStreamSynchronize(s));
+  return MFE_OK;
+}
+
+int mfe_make_local_matrix(mfe_ctx* ctx, mfe_csr* A) {
+  if (!ctx || !A) return MFE_BAD_ARG;
+  // One rank (make_local_matrix.hpp:437-441): every column is local.
+  A->num_cols = A->nrows;
+  A->has_local_indices = true;
+  return MFE_OK;
+}
+
+// ---- Vector ----------------------------------------------------------------
+
+int mfe_vec_create(mfe_ctx* ctx, int64_t n, mfe_vec** out) {
+  if (!ctx || !out || n < 0) return MFE_BAD_ARG;
+  *out = nullptr;
+  MFE_CHECK(ctx, hipSetDevice(ctx->device));
+  mfe_vec* v = new mfe_vec();
+  v->ctx = ctx;
+  v->n = n;
+  hipError_t e = hipMalloc(&v->d, sizeof(double) * (size_t)(n > 0 ? n : 1));
+  if (e != hipSuccess) {
+    delete v;
+    return hip_fail(ctx, e, "hipMalloc(vector)");
+  }
+  e = hipMemsetAsync(v->d, 0, sizeof(double) * (size_t)(n > 0 ? n : 1), ctx->stream);
+  if (e != hipSuccess) {
+    hipFree(v->d);
+    delete v;
+    return hip_fail(ctx, e, "hipMemsetAsync(vector)");
+  }
+  *out = v;
+  return MFE_OK;
+}
+
+void mfe_vec_destroy(mfe_vec* v) {
+  if (!v) return;
+  hipSetDevice(v->ctx->device);
+  hipStreamSynchronize(v->ctx->stream);
+  hipFree(v->d);
+  delete v;
+}
+
+int64_t mfe_vec_size(const mfe_vec* v) { return v ? v->n : -1; }
+
+int mfe_vec_upload(mfe_ctx* ctx, mfe_vec* v, const double* host, int64_t n) {
+  if (!ctx || !v || (n > 0 && !host) || n != v->n) return MFE_BAD_ARG;
+  if (n) MFE_CHECK(ctx, hipMemcpyAsync(v->d, host, sizeof(double) * n, hipMemcpyHostToDevice,
+                                       ctx->stream));
+  MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return MFE_OK;
+}
+
+int mfe_vec_download(mfe_ctx* ctx, const mfe_vec* v, double* host, int64_t n) {
+  if (!ctx || !v || (n > 0 && !host) || n != v->n) return MFE_BAD_ARG;
+  if (n) MFE_CHECK(ctx, hipMemcpyAsync(host, v->d, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                       ctx->stream));
+  MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return MFE_OK;
+}
+
+}  // extern "C"
+
+namespace mfe {
+__global__ void fill_kernel(int64_t n, double v, double* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = v;
+}
+}  // namespace mfe
+
+extern "C" {
+
+int mfe_vec_fill(mfe_ctx* ctx, mfe_vec* v, double value) {
+  if (!ctx || !v) return MFE_BAD_ARG;
+  if (v->n == 0) return MFE_OK;
+  fill_kernel<<<(unsigned)cdiv(v->n, kBlock), kBlock, 0, ctx->stream>>>(v->n, value, v->d);
+  MFE_LAUNCHED(ctx, "fill_kernel");
+  return MFE_OK;
+}
+
+// ---- verification (host) -----------------------------------------------------
+
+// analytic_soln.hpp:51-112: Carslaw & Jaeger series for u=1 on x=1, 0 elsewhere.
+double mfe_soln(double x, double y, double z, int max_p, int max_q) {
+  const double PI = 3.141592653589793238462;
+  const double PI_SQR = PI * PI;
+  const double term0 = 16.0 / (PI_SQR);
+  double sum = 0;
+  for (int p = 0; p <= max_p; ++p) {
+    const double p21y = (2 * p + 1) * PI * y;
+    const double sin_py = std::sin(p21y) / (2 * p + 1);
+    for (int q = 0; q <= max_q; ++q) {
+      const double q21z = (2 * q + 1) * PI * z;
+      const double sin_qz = std::sin(q21z) / (2 * q + 1);
+      const double l =
+          std::sqrt((2 * p + 1) * (2 * p + 1) * PI_SQR + (2 * q + 1) * (2 * q + 1) * PI_SQR);
+      const double sinh1 = std::sinh(l * x);
+      const double sinh2 = std::sinh(l);
+      const double tmp = (sinh1 * sin_py) * (sin_qz / sinh2);
+      if (tmp == tmp) sum += tmp;
+      else break;
+    }
+  }
+  return term0 * sum;
+}
+
+// verify_solution.hpp:55-174.  Only the checked nodes are copied to the host.
+int mfe_verify_solution(mfe_ctx* ctx, const int g[6], const mfe_vec* xv, double tolerance,
+                        int whole, double* max_err) {
+  if (!ctx || !g || !xv) return MFE_BAD_ARG;
+  const int gx = g[1] + 1, gy = g[3] + 1, gz = g[5] + 1;
+  if (g[0] != 0 || g[2] != 0 || g[4] != 0 || (int64_t)gx * gy * gz != xv->n)
+    return fail(ctx, MFE_BAD_ARG, "verify_solution: box does not match x");
+  const int xdiv = gx > 1 ? gx - 1 : 1, ydiv = gy > 1 ? gy - 1 : 1, zdiv = gz > 1 ? gz - 1 : 1;
+  MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<double> host;
+  if (whole) {
+    host.resize((size_t)xv->n);
+    int st = mfe_vec_download(ctx, xv, host.data(), xv->n);
+    if (st) return st;
+  }
+  double maxe = 0.0;
+  for (int iz = 0; iz < gz; ++iz) {
+    double z = (1.0 * iz) / zdiv;
+    if (!whole && !(std::abs(z - 0.5) < 0.05)) continue;
+    for (int iy = 0; iy < gy; ++iy) {
+      double y = 1.0 * iy / ydiv;
+      if (!whole && !(std::abs(y - 0.5) < 0.05)) continue;
+      for (int ix = 0; ix < gx; ++ix) {
+        double x = 1.0 * ix / xdiv;
+        if (!whole && !(std::abs(x - 0.5) < 0.05)) continue;
+        int64_t id = ix + (int64_t)gx * iy + (int64_t)gx * gy * iz;
+        double computed;
+        if (whole) {
+          computed = host[(size_t)id];
+        } else {
+          MFE_CHECK(ctx, hipMemcpy(&computed, xv->d + id, sizeof(double), hipMemcpyDeviceToHost));
+        }
+        double a;
+        if (x == 1.0) a = 1;
+        else if (x == 0.0 || y == 0.0 || z == 0.0) a = 0;
+        else if (y == 1.0 || z == 1.0) a = 0;
+        else a = mfe_soln(x, y, z, 300, 300);
+        double err = std::abs(a - computed);
+        if (err > maxe) maxe = err;
+      }
+    }
+  }
+  if (max_err) *max_err = maxe;
+  return maxe > tolerance ? 1 : 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,454 @@
+// Assembly of the structured hex8 box on gfx950: CSR structure, per-element
+// diffusion matrix + source vector, deterministic gather into CSR, Dirichlet.
+//
+// Bit-exactness: every entry A[r][c] of the reference is the sum, from 0.0 and
+// in ascending element id, of K_e[loc(r)][loc(c)] over the elements sharing
+// nodes r and c (perform_element_loop.hpp:78-100 iterates elements in
+// ascending id; sum_into_row adds one term per element).  The gather kernel
+// below owns one row per thread and visits that row's (up to 8) elements in
+// ascending id, so it performs the same additions in the same order.  The
+// element kernel evaluates Hex8::diffusionMatrix_symm / sourceVector with the
+// reference's operation order; -ffp-contract=off keeps every product rounded.
+#include <cmath>
+
+#include "mfe_common.hpp"
+
+namespace mfe {
+
+int csr_alloc(mfe_ctx* ctx, mfe_csr* A, int64_t nrows, int64_t nnz);  // mfe_api.hip
+
+// Gauss-point tables, computed on the host with the reference's expressions
+// (gauss_pts.hpp:37-62, Hex8.hpp:43-113, Hex8_ElemData.hpp:37-61) and passed
+// by value (read through the scalar cache by every lane).
+struct HexTables {
+  double gv[192];  // grad_vals[gp][node][dim], gp in (ig,jg,kg) order
+  double psi[64];  // shape_fns[gp][node]
+  double gw[2];    // Gauss weights
+};
+
+static void host_gauss_pts(double pts[2], double wts[2]) {
+  const double x2 = 0.577350269;  // truncated 1/sqrt(3), gauss_pts.hpp:41
+  pts[0] = -x2;
+  wts[0] = 1.0;
+  pts[1] = x2;
+  wts[1] = 1.0;
+}
+
+static void host_tables(HexTables& t) {
+  double gp[2], gw[2];
+  host_gauss_pts(gp, gw);
+  t.gw[0] = gw[0];
+  t.gw[1] = gw[1];
+  int q = 0;
+  for (int ig = 0; ig < 2; ++ig)
+    for (int jg = 0; jg < 2; ++jg)
+      for (int kg = 0; kg < 2; ++kg, ++q) {
+        const double x0 = gp[ig], x1 = gp[jg], x2 = gp[kg];
+        const double u = 1.0 - x0, v = 1.0 - x1, w = 1.0 - x2;
+        const double up1 = 1.0 + x0, vp1 = 1.0 + x1, wp1 = 1.0 + x2;
+        double* g = t.gv + 24 * q;
+        g[0] = -0.125 * v * w;      g[1] = -0.125 * u * w;      g[2] = -0.125 * u * v;
+        g[3] = 0.125 * v * w;       g[4] = -0.125 * up1 * w;    g[5] = -0.125 * up1 * v;
+        g[6] = 0.125 * vp1 * w;     g[7] = 0.125 * up1 * w;     g[8] = -0.125 * up1 * vp1;
+        g[9] = -0.125 * vp1 * w;    g[10] = 0.125 * u * w;      g[11] = -0.125 * u * vp1;
+        g[12] = -0.125 * v * wp1;   g[13] = -0.125 * u * wp1;   g[14] = 0.125 * u * v;
+        g[15] = 0.125 * v * wp1;    g[16] = -0.125 * up1 * wp1; g[17] = 0.125 * up1 * v;
+        g[18] = 0.125 * vp1 * wp1;  g[19] = 0.125 * up1 * wp1;  g[20] = 0.125 * up1 * vp1;
+        g[21] = -0.125 * vp1 * wp1; g[22] = 0.125 * u * wp1;    g[23] = 0.125 * u * vp1;
+        double* s = t.psi + 8 * q;
+        s[0] = 0.125 * u * v * w;
+        s[1] = 0.125 * up1 * v * w;
+        s[2] = 0.125 * up1 * vp1 * w;
+        s[3] = 0.125 * u * vp1 * w;
+        s[4] = 0.125 * u * v * wp1;
+        s[5] = 0.125 * up1 * v * wp1;
+        s[6] = 0.125 * up1 * vp1 * wp1;
+        s[7] = 0.125 * u * vp1 * wp1;
+      }
+}
+
+// Exodus hex8 corner layout (Hex8_box_utils.hpp:41-57).
+__host__ __device__ constexpr int corner_x(int c) { return (c == 1 || c == 2 || c == 5 || c == 6); }
+__host__ __device__ constexpr int corner_y(int c) { return (c == 2 || c == 3 || c == 6 || c == 7); }
+__host__ __device__ constexpr int corner_z(int c) { return c >= 4; }
+__host__ __device__ constexpr int corner_of(int x, int y, int z) {
+  return z * 4 + (y ? (x ? 2 : 3) : (x ? 1 : 0));
+}
+// Packed upper triangle index of (m, n), row-major m <= n (Hex8.hpp:324-341).
+__host__ __device__ constexpr int packed(int a, int b) {
+  return a <= b ? a * 8 - a * (a - 1) / 2 + (b - a) : b * 8 - b * (b - 1) / 2 + (a - b);
+}
+
+// Hex8::diffusionMatrix_symm (Hex8.hpp:259-361) and Hex8::sourceVector
+// (Hex8.hpp:363-410) for one element; J, detJ are evaluated once per Gauss
+// point and shared (the reference evaluates the same expressions twice).
+__device__ __forceinline__ void hex8_element(const double c[24], const HexTables& tab, double k,
+                                             double K[36], double F[8]) {
+#pragma unroll
+  for (int i = 0; i < 36; ++i) K[i] = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) F[i] = 0;
+  int q = 0;
+#pragma unroll
+  for (int ig = 0; ig < 2; ++ig) {
+    const double wi = tab.gw[ig];
+#pragma unroll
+    for (int jg = 0; jg < 2; ++jg) {
+      const double wj = tab.gw[jg];
+      const double wi_wj = wi * wj;
+#pragma unroll
+      for (int kg = 0; kg < 2; ++kg, ++q) {
+        const double wk = tab.gw[kg];
+        const double wi_wj_wk = wi_wj * wk;
+        const double* g = tab.gv + 24 * q;
+        double J00 = 0, J01 = 0, J02 = 0, J10 = 0, J11 = 0, J12 = 0, J20 = 0, J21 = 0, J22 = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const double g0 = g[3 * i], g1 = g[3 * i + 1], g2 = g[3 * i + 2];
+          const double x = c[3 * i], y = c[3 * i + 1], z = c[3 * i + 2];
+          J00 += g0 * x; J01 += g0 * y; J02 += g0 * z;
+          J10 += g1 * x; J11 += g1 * y; J12 += g1 * z;
+          J20 += g2 * x; J21 += g2 * y; J22 += g2 * z;
+        }
+        const double term0 = J22 * J11 - J21 * J12;
+        const double term1 = J22 * J01 - J21 * J02;
+        const double term2 = J12 * J01 - J11 * J02;
+        const double detJ = J00 * term0 - J10 * term1 + J20 * term2;
+        const double inv_detJ = 1.0 / detJ;
+        const double i0 = term0 * inv_detJ;
+        const double i1 = -term1 * inv_detJ;
+        const double i2 = term2 * inv_detJ;
+        const double i3 = -(J22 * J10 - J20 * J12) * inv_detJ;
+        const double i4 = (J22 * J00 - J20 * J02) * inv_detJ;
+        const double i5 = -(J12 * J00 - J10 * J02) * inv_detJ;
+        const double i6 = (J21 * J10 - J20 * J11) * inv_detJ;
+        const double i7 = -(J21 * J00 - J20 * J01) * inv_detJ;
+        const double i8 = (J11 * J00 - J10 * J01) * inv_detJ;
+        const double kdw = k * detJ * wi_wj_wk;
+        double dx[8], dy[8], dz[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const double g0 = g[3 * i], g1 = g[3 * i + 1], g2 = g[3 * i + 2];
+          dx[i] = g0 * i0 + g1 * i1 + g2 * i2;
+          dy[i] = g0 * i3 + g1 * i4 + g2 * i5;
+          dz[i] = g0 * i6 + g1 * i7 + g2 * i8;
+        }
+        int off = 0;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          K[off++] += kdw * ((dx[m] * dx[m]) + (dy[m] * dy[m]) + (dz[m] * dz[m]));
+#pragma unroll
+          for (int n = m + 1; n < 8; ++n)
+            K[off++] += kdw * ((dx[m] * dx[n]) + (dy[m] * dy[n]) + (dz[m] * dz[n]));
+        }
+        // sourceVector: term = Q*detJ*wi*wj*wk with Q = 1.0 (Hex8.hpp:379,402).
+        const double term = 1.0 * detJ * wi * wj * wk;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) F[i] += tab.psi[8 * q + i] * term;
+      }
+    }
+  }
+}
+
+// One thread per element e (x fastest, ascending id).  Coordinates follow
+// get_elem_nodes_and_coords (Hex8_box_utils.hpp:101-159): node 0 from
+// get_coords (box_utils.hpp:64-82), the others node0 + h.
+// Output layout: K[q * ne + e], F[q * ne + e] (structure of arrays, so the
+// gather kernel's neighbouring threads read neighbouring addresses).
+__global__ __launch_bounds__(kBlock) void element_kernel(int Nx, int Ny, int Nz, double hx,
+                                                         double hy, double hz, HexTables tab,
+                                                         const double* __restrict__ kappa,
+                                                         double* __restrict__ Kout,
+                                                         double* __restrict__ Fout, int64_t ne) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int ex = (int)(e % Nx);
+  const int ey = (int)((e / Nx) % Ny);
+  const int ez = (int)(e / ((int64_t)Nx * Ny));
+  const double x0 = 1.0 * ex / Nx;
+  const double y0 = 1.0 * ey / Ny;
+  const double z0 = (1.0 * ez) / Nz;
+  const double x1 = x0 + hx, y1 = y0 + hy, z1 = z0 + hz;
+  const double c[24] = {x0, y0, z0, x1, y0, z0, x1, y1, z0, x0, y1, z0,
+                        x0, y0, z1, x1, y0, z1, x1, y1, z1, x0, y1, z1};
+  double K[36], F[8];
+  hex8_element(c, tab, kappa ? kappa[e] : 1.0, K, F);
+#pragma unroll
+  for (int i = 0; i < 36; ++i) Kout[(int64_t)i * ne + e] = K[i];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) Fout[(int64_t)i * ne + e] = F[i];
+}
+
+// One thread per node (row): sum the row's element contributions in ascending
+// element id onto the existing values (sum_into_row adds, it does not store),
+// then store the row's 8..27 entries in ascending column order.
+__global__ __launch_bounds__(kBlock) void gather_kernel(int Nx, int Ny, int Nz,
+                                                        const int* __restrict__ ro,
+                                                        const double* __restrict__ K,
+                                                        const double* __restrict__ F, int64_t ne,
+                                                        double* __restrict__ vals,
+                                                        double* __restrict__ b) {
+  const int gx = Nx + 1, gy = Ny + 1;
+  const int64_t n = (int64_t)gx * gy * (Nz + 1);
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const int ix = (int)(id % gx);
+  const int iy = (int)((id / gx) % gy);
+  const int iz = (int)(id / ((int64_t)gx * gy));
+  const int lox = ix > 0 ? -1 : 0, hix = ix < Nx ? 1 : 0;
+  const int loy = iy > 0 ? -1 : 0, hiy = iy < Ny ? 1 : 0;
+  const int loz = iz > 0 ? -1 : 0, hiz = iz < Nz ? 1 : 0;
+  const int64_t base = ro[id];
+
+  double acc[27];
+  {
+    int pos = 0;
+#pragma unroll
+    for (int sz = -1; sz <= 1; ++sz)
+#pragma unroll
+      for (int sy = -1; sy <= 1; ++sy)
+#pragma unroll
+        for (int sx = -1; sx <= 1; ++sx) {
+          const int s = (sz + 1) * 9 + (sy + 1) * 3 + (sx + 1);
+          const bool in = sx >= lox && sx <= hix && sy >= loy && sy <= hiy && sz >= loz && sz <= hiz;
+          acc[s] = in ? vals[base + pos] : 0.0;
+          pos += in ? 1 : 0;
+        }
+  }
+  double accb = b[id];
+
+#pragma unroll
+  for (int dz = -1; dz <= 0; ++dz)
+#pragma unroll
+    for (int dy = -1; dy <= 0; ++dy)
+#pragma unroll
+      for (int dx = -1; dx <= 0; ++dx) {
+        const int ex = ix + dx, ey = iy + dy, ez = iz + dz;
+        if (ex < 0 || ex >= Nx || ey < 0 || ey >= Ny || ez < 0 || ez >= Nz) continue;
+        const int64_t e = ex + (int64_t)Nx * (ey + (int64_t)Ny * ez);
+        const int cx = -dx, cy = -dy, cz = -dz;  // this node's corner in element e
+        const int li = corner_of(cx, cy, cz);
+#pragma unroll
+        for (int lj = 0; lj < 8; ++lj) {
+          const int s = (corner_z(lj) - cz + 1) * 9 + (corner_y(lj) - cy + 1) * 3 +
+                        (corner_x(lj) - cx + 1);
+          acc[s] += K[(int64_t)packed(li, lj) * ne + e];
+        }
+        accb += F[(int64_t)li * ne + e];
+      }
+
+  int pos = 0;
+#pragma unroll
+  for (int sz = -1; sz <= 1; ++sz)
+#pragma unroll
+    for (int sy = -1; sy <= 1; ++sy)
+#pragma unroll
+      for (int sx = -1; sx <= 1; ++sx) {
+        const int s = (sz + 1) * 9 + (sy + 1) * 3 + (sx + 1);
+        const bool in = sx >= lox && sx <= hix && sy >= loy && sy <= hiy && sz >= loz && sz <= hiz;
+        if (in) vals[base + pos] = acc[s];
+        pos += in ? 1 : 0;
+      }
+  b[id] = accb;
+}
+
+// generate_matrix_structure + MatrixInitOp: one thread per node.  The row
+// offset has a closed form: per axis a node has cnt(i) = 2 (ends) or 3
+// in-range neighbours and the running count before index i is 3i-1 (i>0).
+__device__ __forceinline__ int64_t axis_cnt(int i, int g) { return 1 + (i > 0) + (i < g - 1); }
+__device__ __forceinline__ int64_t axis_pre(int i) { return i > 0 ? 3 * (int64_t)i - 1 : 0; }
+
+__global__ __launch_bounds__(kBlock) void structure_kernel(int gx, int gy, int gz,
+                                                           int* __restrict__ rows,
+                                                           int* __restrict__ ro,
+                                                           int* __restrict__ cols) {
+  const int64_t n = (int64_t)gx * gy * gz;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const int ix = (int)(id % gx);
+  const int iy = (int)((id / gx) % gy);
+  const int iz = (int)(id / ((int64_t)gx * gy));
+  const int64_t Sx = 3 * (int64_t)gx - 2, Sy = 3 * (int64_t)gy - 2;
+  const int64_t cz = axis_cnt(iz, gz), cy = axis_cnt(iy, gy);
+  const int64_t off = axis_pre(iz) * Sy * Sx + cz * axis_pre(iy) * Sx + cz * cy * axis_pre(ix);
+  rows[id] = (int)id;
+  ro[id] = (int)off;
+  if (id == n - 1) ro[n] = (int)(off + cz * cy * axis_cnt(ix, gx));
+  int64_t pos = off;
+#pragma unroll
+  for (int sz = -1; sz <= 1; ++sz)
+#pragma unroll
+    for (int sy = -1; sy <= 1; ++sy)
+#pragma unroll
+      for (int sx = -1; sx <= 1; ++sx) {
+        const int x = ix + sx, y = iy + sy, z = iz + sz;
+        if (x < 0 || y < 0 || z < 0 || x >= gx || y >= gy || z >= gz) continue;
+        cols[pos++] = (int)(x + (int64_t)gx * y + (int64_t)gx * gy * z);
+      }
+}
+
+__global__ void bc_flag_kernel(int64_t nbc, const int* __restrict__ bc, int64_t n,
+                               unsigned char* __restrict__ flags) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nbc) return;
+  const int r = bc[k];
+  if (r >= 0 && r < n) flags[r] = 1;
+}
+
+// impose_dirichlet (SparseMatrix_functions.hpp:431-480), one thread per row.
+// BC rows: b = value and identity row (zero_row_and_put_1_on_diagonal).
+// Other rows: sum the BC-column coefficients in column order, zero them, and
+// b -= sum * value.  Rows are independent, so one pass does both loops.
+__global__ __launch_bounds__(kBlock) void dirichlet_kernel(int64_t n, const int* __restrict__ ro,
+                                                           const int* __restrict__ cols,
+                                                           double* __restrict__ vals,
+                                                           double* __restrict__ b,
+                                                           const unsigned char* __restrict__ flags,
+                                                           double value) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  const int j0 = ro[row], j1 = ro[row + 1];
+  if (flags[row]) {
+    b[row] = value;
+    for (int j = j0; j < j1; ++j) vals[j] = (cols[j] == row) ? 1.0 : 0.0;
+    return;
+  }
+  double sum = 0;
+  for (int j = j0; j < j1; ++j) {
+    if (flags[cols[j]]) {
+      sum += vals[j];
+      vals[j] = 0;
+    }
+  }
+  b[row] -= sum * value;
+}
+
+static bool box_ok(const int* g) {
+  return g && g[0] == 0 && g[2] == 0 && g[4] == 0 && g[1] > 0 && g[3] > 0 && g[5] > 0;
+}
+static bool same_box(const int* a, const int* b) {
+  for (int i = 0; i < 6; ++i)
+    if (a[i] != b[i]) return false;
+  return true;
+}
+
+}  // namespace mfe
+
+using namespace mfe;
+
+extern "C" {
+
+int mfe_generate_matrix_structure(mfe_ctx* ctx, const int gbox[6], const int lbox[6],
+                                  mfe_csr* A) {
+  if (!ctx || !A || !box_ok(gbox) || !lbox) return MFE_BAD_ARG;
+  if (!same_box(gbox, lbox))
+    return fail(ctx, MFE_UNSUPPORTED, "one rank: local box must equal the global box");
+  MFE_CHECK(ctx, hipSetDevice(ctx->device));
+  const int gx = gbox[1] + 1, gy = gbox[3] + 1, gz = gbox[5] + 1;
+  const int64_t n = (int64_t)gx * gy * gz;
+  const int64_t nnz = (3 * (int64_t)gx - 2) * (3 * (int64_t)gy - 2) * (3 * (int64_t)gz - 2);
+  int st = csr_alloc(ctx, A, n, nnz);
+  if (st) return st;
+  structure_kernel<<<(unsigned)cdiv(n, kBlock), kBlock, 0, ctx->stream>>>(gx, gy, gz, A->rows,
+                                                                          A->row_offsets, A->cols);
+  MFE_LAUNCHED(ctx, "structure_kernel");
+  MFE_CHECK(ctx, hipMemsetAsync(A->vals, 0, sizeof(double) * (size_t)nnz, ctx->stream));
+  A->has_local_indices = false;
+  A->num_cols = 0;
+  A->max_block_nnz = -1;
+  return plan_spmv(ctx, A);
+}
+
+int mfe_perform_element_loop(mfe_ctx* ctx, const int gbox[6], const int ebox[6], mfe_csr* A,
+                             mfe_vec* b, const mfe_vec* kappa) {
+  if (!ctx || !A || !b || !box_ok(gbox) || !ebox) return MFE_BAD_ARG;
+  if (!same_box(gbox, ebox))
+    return fail(ctx, MFE_UNSUPPORTED, "one rank: the element box must be the global box");
+  MFE_CHECK(ctx, hipSetDevice(ctx->device));
+  const int Nx = gbox[1], Ny = gbox[3], Nz = gbox[5];
+  const int64_t n = (int64_t)(Nx + 1) * (Ny + 1) * (Nz + 1);
+  const int64_t nnz = (3 * (int64_t)Nx + 1) * (3 * (int64_t)Ny + 1) * (3 * (int64_t)Nz + 1);
+  const int64_t ne = (int64_t)Nx * Ny * Nz;
+  if (A->nrows != n || A->nnz != nnz || b->n != n)
+    return fail(ctx, MFE_BAD_ARG, "A/b do not hold this box's structure");
+  if (kappa && kappa->n != ne) return fail(ctx, MFE_BAD_ARG, "kappa must have one value per element");
+  HexTables tab;
+  host_tables(tab);
+  double *Kb = nullptr, *Fb = nullptr;
+  hipError_t e1 = hipMalloc(&Kb, sizeof(double) * 36 * (size_t)ne);
+  if (e1 != hipSuccess) return hip_fail(ctx, e1, "hipMalloc(element matrices)");
+  e1 = hipMalloc(&Fb, sizeof(double) * 8 * (size_t)ne);
+  if (e1 != hipSuccess) {
+    hipFree(Kb);
+    return hip_fail(ctx, e1, "hipMalloc(element vectors)");
+  }
+  int st = MFE_OK;
+  element_kernel<<<(unsigned)cdiv(ne, kBlock), kBlock, 0, ctx->stream>>>(
+      Nx, Ny, Nz, 1.0 / Nx, 1.0 / Ny, 1.0 / Nz, tab, kappa ? kappa->d : nullptr, Kb, Fb, ne);
+  hipError_t le = hipGetLastError();
+  if (le == hipSuccess) {
+    gather_kernel<<<(unsigned)cdiv(n, kBlock), kBlock, 0, ctx->stream>>>(Nx, Ny, Nz,
+                                                                          A->row_offsets, Kb, Fb,
+                                                                          ne, A->vals, b->d);
+    le = hipGetLastError();
+  }
+  if (le != hipSuccess) st = hip_fail(ctx, le, "element/gather kernels");
+  hipError_t se = hipStreamSynchronize(ctx->stream);
+  if (st == MFE_OK && se != hipSuccess) st = hip_fail(ctx, se, "assembly");
+  hipFree(Kb);
+  hipFree(Fb);
+  return st;
+}
+
+int mfe_assemble_FE_data(mfe_ctx* ctx, const int gbox[6], const int lbox[6], mfe_csr* A,
+                         mfe_vec* b, const mfe_vec* kappa) {
+  if (!ctx || !box_ok(gbox) || !lbox) return MFE_BAD_ARG;
+  if (!same_box(gbox, lbox))
+    return fail(ctx, MFE_UNSUPPORTED, "one rank: local box must equal the global box");
+  // One rank: the ghost layer of assemble_FE_data.hpp:59-72 is empty.
+  return mfe_perform_element_loop(ctx, gbox, lbox, A, b, kappa);
+}
+
+int mfe_impose_dirichlet(mfe_ctx* ctx, double value, mfe_csr* A, mfe_vec* b, int gnx, int gny,
+                         int gnz, const int* bc_rows, int64_t nbc) {
+  (void)gnx;
+  (void)gny;
+  (void)gnz;
+  if (!ctx || !A || !b || nbc < 0 || (nbc > 0 && !bc_rows)) return MFE_BAD_ARG;
+  if (b->n != A->nrows) return fail(ctx, MFE_BAD_ARG, "b length != rows");
+  MFE_CHECK(ctx, hipSetDevice(ctx->device));
+  const int64_t n = A->nrows;
+  if (n == 0) return MFE_OK;
+  int* d_bc = nullptr;
+  unsigned char* flags = nullptr;
+  hipError_t e = hipMalloc(&d_bc, sizeof(int) * (size_t)(nbc > 0 ? nbc : 1));
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc(bc rows)");
+  e = hipMalloc(&flags, (size_t)n);
+  if (e != hipSuccess) {
+    hipFree(d_bc);
+    return hip_fail(ctx, e, "hipMalloc(bc flags)");
+  }
+  int st = MFE_OK;
+  do {
+    if (nbc > 0) {
+      e = hipMemcpyAsync(d_bc, bc_rows, sizeof(int) * nbc, hipMemcpyHostToDevice, ctx->stream);
+      if (e != hipSuccess) { st = hip_fail(ctx, e, "bc rows H2D"); break; }
+    }
+    e = hipMemsetAsync(flags, 0, (size_t)n, ctx->stream);
+    if (e != hipSuccess) { st = hip_fail(ctx, e, "bc flags memset"); break; }
+    if (nbc > 0) {
+      bc_flag_kernel<<<(unsigned)cdiv(nbc, kBlock), kBlock, 0, ctx->stream>>>(nbc, d_bc, n, flags);
+      if ((e = hipGetLastError()) != hipSuccess) { st = hip_fail(ctx, e, "bc_flag_kernel"); break; }
+    }
+    dirichlet_kernel<<<(unsigned)cdiv(n, kBlock), kBlock, 0, ctx->stream>>>(
+        n, A->row_offsets, A->cols, A->vals, b->d, flags, value);
+    if ((e = hipGetLastError()) != hipSuccess) { st = hip_fail(ctx, e, "dirichlet_kernel"); break; }
+  } while (0);
+  hipError_t se = hipStreamSynchronize(ctx->stream);
+  if (st == MFE_OK && se != hipSuccess) st = hip_fail(ctx, se, "impose_dirichlet");
+  hipFree(d_bc);
+  hipFree(flags);
+  return st;
+}
+
+}  // extern "C"

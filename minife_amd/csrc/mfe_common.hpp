@@ -1,0 +1,115 @@
+// Internal definitions shared by the libminife_hip.so translation units.
+// Everything here is compiled for gfx950 with -ffp-contract=off, so each
+// a*b+c of the reference stays two rounded operations (bit-identical results).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "minife_hip.h"
+
+namespace mfe {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kBlock = 256;        // threads per workgroup (4 waves)
+constexpr int kNumXcd = 8;         // MI355X: 8 XCDs, one L2 each
+constexpr int kSpmvRows = 256;     // rows per SpMV workgroup (one per thread)
+constexpr int kSpmvMaxNnz = 27 * kSpmvRows;  // LDS product slots per workgroup
+constexpr int kVecBlocks = 2048;   // fixed grid of the vector kernels
+
+struct CgWorkspace;  // mfe_solve.hip
+
+}  // namespace mfe
+
+struct mfe_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t t_start = nullptr, t_stop = nullptr;
+  std::string err;
+  double* d_partials = nullptr;  // reduction partials (3 x kPartialsCap)
+  size_t partials_cap = 0;
+  double* d_scalars = nullptr;   // small device scalar area
+  double* h_scalars = nullptr;   // pinned host mirror
+  mfe::CgWorkspace* cg = nullptr;
+  int cu_count = 0;
+};
+
+struct mfe_csr {
+  mfe_ctx* ctx = nullptr;
+  int64_t nrows = 0, nnz = 0;
+  int* rows = nullptr;
+  int* row_offsets = nullptr;
+  int* cols = nullptr;
+  double* vals = nullptr;
+  bool has_local_indices = false;
+  int64_t num_cols = 0;
+  // SpMV plan: largest nnz of any kSpmvRows-row block (-1 = not computed).
+  int64_t max_block_nnz = -1;
+};
+
+struct mfe_vec {
+  mfe_ctx* ctx = nullptr;
+  int64_t n = 0;
+  double* d = nullptr;
+};
+
+namespace mfe {
+
+int fail(mfe_ctx* ctx, int status, const std::string& msg);
+int hip_fail(mfe_ctx* ctx, hipError_t e, const char* what);
+// Recompute A->max_block_nnz on the device (synchronises the stream).
+int plan_spmv(mfe_ctx* ctx, mfe_csr* A);
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace mfe
+
+#define MFE_CHECK(ctx, call)                                       \
+  do {                                                             \
+    hipError_t e_ = (call);                                        \
+    if (e_ != hipSuccess) return ::mfe::hip_fail((ctx), e_, #call); \
+  } while (0)
+
+#define MFE_LAUNCHED(ctx, what)                                     \
+  do {                                                              \
+    hipError_t e_ = hipGetLastError();                              \
+    if (e_ != hipSuccess) return ::mfe::hip_fail((ctx), e_, (what)); \
+  } while (0)
+
+namespace mfe {
+
+// Workgroup i of a launch is dispatched to XCD (i mod 8).  This maps the
+// dispatch index to a logical tile index so that each XCD sweeps one
+// contiguous range of tiles: neighbouring rows then share their x-vector
+// lines in the same L2.  A bijection on [0, n).
+__device__ __forceinline__ int xcd_contiguous(int bid, int n) {
+  const int q = n / kNumXcd, r = n % kNumXcd;
+  const int xcd = bid % kNumXcd, local = bid / kNumXcd;
+  return xcd * q + (xcd < r ? xcd : r) + local;
+}
+
+// Wave64 sum in a fixed butterfly order (deterministic).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Workgroup sum for kBlock threads: wave butterflies, then the four wave
+// totals added in wave order by thread 0.  Result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* lds4) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  if (lane == 0) lds4[wave] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x == 0) t = ((lds4[0] + lds4[1]) + lds4[2]) + lds4[3];
+  __syncthreads();
+  return t;
+}
+
+}  // namespace mfe

@@ -1,0 +1,226 @@
+"""ctypes binding of the C restatement of miniFE ref/ (liboracle_minife.so).
+
+TEST INFRASTRUCTURE ONLY: the parity oracle for the HIP library and the
+"port" CPU baseline.  Only tests/, ``__graft_entry__.smoke`` and bench.py's
+CPU-baseline leg import this module, always as the checker, never as the
+thing measured or shipped.  The product (``minife_amd``) never imports it.
+
+Functions mirror the reference operators (see minife_oracle.h for the
+file:line each restates).  Arrays are numpy; indices int32, values float64.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle_minife.so")
+REF_DUMP = os.path.join(_HERE, "_ref", "ref_dump")
+REF_TIME = os.path.join(_HERE, "_ref", "ref_time")
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i64 = ctypes.c_int64
+_lib = None
+
+
+def build(force: bool = False) -> None:
+    """Compile liboracle_minife.so (and oracle/_ref when the reference exists)."""
+    targets = ["oracle"]
+    if os.path.isdir("/root/reference/ref"):
+        targets.append("ref")
+    if force:
+        for p in (LIB_PATH, REF_DUMP, REF_TIME):
+            if os.path.exists(p):
+                os.remove(p)
+    subprocess.run(["make", "-s", "-C", _HERE] + targets, check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle oracle`)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.orc_structure_sizes.argtypes = [_i32p, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]
+    L.orc_generate_matrix_structure.argtypes = [_i32p, _i32p, _i32p, _i32p, _f64p]
+    L.orc_compute_gradient_values.argtypes = [_f64p]
+    L.orc_diffusion_matrix_symm.argtypes = [_f64p, _f64p, ctypes.c_double, _f64p]
+    L.orc_source_vector.argtypes = [_f64p, _f64p, ctypes.c_double, _f64p]
+    L.orc_get_elem_nodes_and_coords.argtypes = [_i32p, ctypes.c_int, _i32p, _f64p]
+    L.orc_assemble_FE_data.argtypes = [_i32p, _i64, _i32p, _i32p, _f64p, _f64p, vp]
+    L.orc_bc_rows.argtypes = [_i32p, ctypes.c_int, vp]
+    L.orc_bc_rows.restype = _i64
+    L.orc_impose_dirichlet.argtypes = [ctypes.c_double, _i64, _i32p, _i32p, _f64p, _f64p,
+                                       _i32p, _i64]
+    L.orc_matvec.argtypes = [_i64, _i32p, _i32p, _f64p, _f64p, _f64p]
+    L.orc_dot.argtypes = [_i64, _f64p, _f64p]
+    L.orc_dot.restype = ctypes.c_double
+    L.orc_waxpby.argtypes = [_i64, ctypes.c_double, _f64p, ctypes.c_double, _f64p, _f64p]
+    L.orc_cg_solve.argtypes = [_i64, _i32p, _i32p, _f64p, _f64p, _f64p, ctypes.c_int,
+                               ctypes.c_double, ctypes.POINTER(ctypes.c_int),
+                               ctypes.POINTER(ctypes.c_double), vp, vp]
+    L.orc_soln.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, ctypes.c_int]
+    L.orc_soln.restype = ctypes.c_double
+    L.orc_verify_solution.argtypes = [_i32p, _f64p, ctypes.c_double, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_double)]
+    _lib = L
+    return L
+
+
+def box(nx: int, ny: int, nz: int) -> np.ndarray:
+    return np.array([0, nx, 0, ny, 0, nz], dtype=np.int32)
+
+
+class CSR:
+    """Host CSR with the fields of ref/src/CSRMatrix.hpp:40-129 (one rank)."""
+
+    def __init__(self, rows, row_offsets, cols, vals):
+        self.rows = rows
+        self.row_offsets = row_offsets
+        self.cols = cols
+        self.vals = vals
+
+    @property
+    def nrows(self) -> int:
+        return int(self.rows.shape[0])
+
+    @property
+    def nnz(self) -> int:
+        return int(self.row_offsets[-1])
+
+    def copy(self) -> "CSR":
+        return CSR(self.rows.copy(), self.row_offsets.copy(), self.cols.copy(), self.vals.copy())
+
+
+def structure_sizes(nx, ny, nz):
+    n, z = _i64(), _i64()
+    if lib().orc_structure_sizes(box(nx, ny, nz), ctypes.byref(n), ctypes.byref(z)):
+        raise ValueError("bad box")
+    return n.value, z.value
+
+
+def generate_matrix_structure(nx, ny, nz) -> CSR:
+    n, nnz = structure_sizes(nx, ny, nz)
+    A = CSR(np.empty(n, np.int32), np.empty(n + 1, np.int32), np.empty(nnz, np.int32),
+            np.empty(nnz, np.float64))
+    lib().orc_generate_matrix_structure(box(nx, ny, nz), A.rows, A.row_offsets, A.cols, A.vals)
+    return A
+
+
+def gradient_values() -> np.ndarray:
+    gv = np.empty(192, np.float64)
+    lib().orc_compute_gradient_values(gv)
+    return gv
+
+
+def element(coords, k: float = 1.0):
+    """(K_e upper triangle [36], f_e [8]) for one hex8 (Hex8.hpp:259-410)."""
+    c = np.ascontiguousarray(coords, dtype=np.float64).reshape(24)
+    gv = gradient_values()
+    K = np.empty(36, np.float64)
+    f = np.empty(8, np.float64)
+    lib().orc_diffusion_matrix_symm(c, gv, float(k), K)
+    lib().orc_source_vector(c, gv, 1.0, f)
+    return K, f
+
+
+def elem_nodes_and_coords(nx, ny, nz, e: int):
+    ids = np.empty(8, np.int32)
+    c = np.empty(24, np.float64)
+    lib().orc_get_elem_nodes_and_coords(box(nx, ny, nz), int(e), ids, c)
+    return ids, c
+
+
+def assemble_FE_data(nx, ny, nz, A: CSR, kappa=None, b=None) -> np.ndarray:
+    """Adds the element contributions onto A.vals and b (a new zero b if None)."""
+    if b is None:
+        b = np.zeros(A.nrows, np.float64)
+    kp = None
+    if kappa is not None:
+        kappa = np.ascontiguousarray(kappa, dtype=np.float64)
+        assert kappa.shape == (nx * ny * nz,)
+        kp = kappa.ctypes.data_as(ctypes.c_void_p)
+    rc = lib().orc_assemble_FE_data(box(nx, ny, nz), A.nrows, A.row_offsets, A.cols, A.vals,
+                                    b, kp)
+    if rc:
+        raise ValueError("assemble failed")
+    return b
+
+
+def bc_rows(nx, ny, nz, which: int) -> np.ndarray:
+    L = lib()
+    cnt = L.orc_bc_rows(box(nx, ny, nz), which, None)
+    out = np.empty(cnt, np.int32)
+    L.orc_bc_rows(box(nx, ny, nz), which, out.ctypes.data_as(ctypes.c_void_p))
+    return out
+
+
+def impose_dirichlet(value: float, A: CSR, b: np.ndarray, bc: np.ndarray) -> None:
+    bc = np.ascontiguousarray(bc, dtype=np.int32)
+    lib().orc_impose_dirichlet(float(value), A.nrows, A.row_offsets, A.cols, A.vals, b, bc,
+                               bc.shape[0])
+
+
+def matvec(A: CSR, x: np.ndarray, y: np.ndarray | None = None) -> np.ndarray:
+    if y is None:
+        y = np.zeros(A.nrows, np.float64)
+    lib().orc_matvec(A.nrows, A.row_offsets, A.cols, A.vals, np.ascontiguousarray(x), y)
+    return y
+
+
+def dot(x, y) -> float:
+    return lib().orc_dot(x.shape[0], np.ascontiguousarray(x), np.ascontiguousarray(y))
+
+
+def waxpby(alpha, x, beta, y, w=None):
+    if w is None:
+        w = np.empty_like(x)
+    lib().orc_waxpby(x.shape[0], float(alpha), x, float(beta), y, w)
+    return w
+
+
+def cg_solve(A: CSR, b: np.ndarray, x: np.ndarray | None = None, max_iter: int = 200,
+             tol: float = float(np.finfo(np.float64).eps), want_times: bool = False):
+    """Returns dict(x, num_iters, normr, history[k,4]={rtrans,pAp,alpha,beta}, status)."""
+    if x is None:
+        x = np.zeros(A.nrows, np.float64)
+    hist = np.zeros((max(max_iter, 1), 4), np.float64)
+    times = np.zeros(5, np.float64)
+    ni = ctypes.c_int(0)
+    nr = ctypes.c_double(0)
+    st = lib().orc_cg_solve(A.nrows, A.row_offsets, A.cols, A.vals, b, x, int(max_iter),
+                            float(tol), ctypes.byref(ni), ctypes.byref(nr),
+                            hist.ctypes.data_as(ctypes.c_void_p),
+                            times.ctypes.data_as(ctypes.c_void_p))
+    out = dict(x=x, num_iters=ni.value, normr=nr.value, history=hist[: ni.value],
+               status=st)
+    if want_times:
+        out["times"] = times
+    return out
+
+
+def soln(x, y, z, max_p=300, max_q=300) -> float:
+    return lib().orc_soln(float(x), float(y), float(z), int(max_p), int(max_q))
+
+
+def verify_solution(nx, ny, nz, x, tol=0.06, whole=False):
+    e = ctypes.c_double(0)
+    rc = lib().orc_verify_solution(box(nx, ny, nz), np.ascontiguousarray(x), float(tol),
+                                   int(whole), ctypes.byref(e))
+    return rc, e.value
+
+
+def build_problem(nx, ny, nz, kappa=None):
+    """Structure -> assembly -> Dirichlet (0 then 1), as driver.hpp:181-218."""
+    A = generate_matrix_structure(nx, ny, nz)
+    b = assemble_FE_data(nx, ny, nz, A, kappa)
+    A_pre, b_pre = A.copy(), b.copy()
+    impose_dirichlet(0.0, A, b, bc_rows(nx, ny, nz, 0))
+    impose_dirichlet(1.0, A, b, bc_rows(nx, ny, nz, 1))
+    return dict(A=A, b=b, A_prebc=A_pre, b_prebc=b_pre)

@@ -1,0 +1,529 @@
+/*
+ * minife_oracle.c -- CPU restatement of miniFE ref/'s assemble-and-solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY (parity oracle + "port" CPU baseline); the product
+ * never links it.  Built with -ffp-contract=off so every a*b+c is two rounded
+ * operations, as in the reference's x86-64 g++ -O3 build.  See minife_oracle.h
+ * for the per-function reference citations and how each one is pinned.
+ */
+#define _POSIX_C_SOURCE 199309L
+#include "minife_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* ref/utils/box_utils.hpp:99-111 (get_id): -1 outside the node box. */
+static int get_id(int nx, int ny, int nz, int x, int y, int z) {
+  if (x < 0 || y < 0 || z < 0) return -1;
+  if (x >= nx || y >= ny || z >= nz) return -1;
+  int tmp = nx * ny;
+  tmp *= z;
+  tmp = x + nx * y + tmp;
+  return tmp;
+}
+
+/* ref/utils/box_utils.hpp:56-62 (get_int_coords) */
+static void get_int_coords(int id, int nx, int ny, int* x, int* y, int* z) {
+  *z = id / (nx * ny);
+  *y = (id % (nx * ny)) / nx;
+  *x = id % nx;
+}
+
+/* ref/utils/box_utils.hpp:64-82 (get_coords) */
+static void get_coords(int id, int nx, int ny, int nz, double* x, double* y, double* z) {
+  const int xdiv = nx > 1 ? nx - 1 : 1;
+  const int ydiv = ny > 1 ? ny - 1 : 1;
+  const int zdiv = nz > 1 ? nz - 1 : 1;
+  *z = (1.0 * (id / (nx * ny))) / zdiv;
+  *y = 1.0 * ((id % (nx * ny)) / nx) / ydiv;
+  *x = 1.0 * (id % nx) / xdiv;
+}
+
+static int box_is_origin(const int g[6]) {
+  return g[0] == 0 && g[2] == 0 && g[4] == 0 && g[1] > 0 && g[3] > 0 && g[5] > 0;
+}
+
+int orc_structure_sizes(const int gbox[6], int64_t* nrows, int64_t* nnz) {
+  if (!box_is_origin(gbox)) return 1;
+  int64_t gx = gbox[1] + 1, gy = gbox[3] + 1, gz = gbox[5] + 1;
+  *nrows = gx * gy * gz;
+  *nnz = (3 * gx - 2) * (3 * gy - 2) * (3 * gz - 2);
+  return 0;
+}
+
+/* ref/src/generate_matrix_structure.hpp:66-125: node box = element box + 1 at
+ * the high end, rows in z->y->x order, row length = in-range 27-stencil
+ * neighbours; MatrixInitOp.hpp:127-156 writes the cols in sz,sy,sx order
+ * (already ascending on one rank, so sort_if_needed is a no-op). */
+int orc_generate_matrix_structure(const int gbox[6], int* rows, int* row_offsets,
+                                  int* cols, double* vals) {
+  if (!box_is_origin(gbox)) return 1;
+  const int gx = gbox[1] + 1, gy = gbox[3] + 1, gz = gbox[5] + 1;
+  int global_nrows = gx;
+  global_nrows *= gy * gz;
+  int roffset = 0;
+  int nnz = 0;
+  for (int iz = 0; iz < gz; ++iz)
+    for (int iy = 0; iy < gy; ++iy)
+      for (int ix = 0; ix < gx; ++ix) {
+        int row_id = get_id(gx, gy, gz, ix, iy, iz);
+        rows[roffset] = row_id;
+        row_offsets[roffset++] = nnz;
+        for (int sz = -1; sz <= 1; ++sz)
+          for (int sy = -1; sy <= 1; ++sy)
+            for (int sx = -1; sx <= 1; ++sx) {
+              int col_id = get_id(gx, gy, gz, ix + sx, iy + sy, iz + sz);
+              if (col_id >= 0 && col_id < global_nrows) {
+                cols[nnz] = col_id;
+                vals[nnz] = 0;
+                ++nnz;
+              }
+            }
+      }
+  row_offsets[roffset] = nnz;
+  return 0;
+}
+
+void orc_gauss_pts(double pts[2], double wts[2]) {
+  const double x2 = 0.577350269; /* gauss_pts.hpp:41, truncated 1/sqrt(3) */
+  pts[0] = -x2;
+  wts[0] = 1.0;
+  pts[1] = x2;
+  wts[1] = 1.0;
+}
+
+void orc_shape_fns(const double* x, double* v) {
+  const double u = 1.0 - x[0], vv = 1.0 - x[1], w = 1.0 - x[2];
+  const double up1 = 1.0 + x[0], vp1 = 1.0 + x[1], wp1 = 1.0 + x[2];
+  v[0] = 0.125 * u * vv * w;
+  v[1] = 0.125 * up1 * vv * w;
+  v[2] = 0.125 * up1 * vp1 * w;
+  v[3] = 0.125 * u * vp1 * w;
+  v[4] = 0.125 * u * vv * wp1;
+  v[5] = 0.125 * up1 * vv * wp1;
+  v[6] = 0.125 * up1 * vp1 * wp1;
+  v[7] = 0.125 * u * vp1 * wp1;
+}
+
+void orc_gradients(const double* x, double* g) {
+  const double u = 1.0 - x[0], v = 1.0 - x[1], w = 1.0 - x[2];
+  const double up1 = 1.0 + x[0], vp1 = 1.0 + x[1], wp1 = 1.0 + x[2];
+  g[0] = -0.125 * v * w;    g[1] = -0.125 * u * w;    g[2] = -0.125 * u * v;
+  g[3] = 0.125 * v * w;     g[4] = -0.125 * up1 * w;  g[5] = -0.125 * up1 * v;
+  g[6] = 0.125 * vp1 * w;   g[7] = 0.125 * up1 * w;   g[8] = -0.125 * up1 * vp1;
+  g[9] = -0.125 * vp1 * w;  g[10] = 0.125 * u * w;    g[11] = -0.125 * u * vp1;
+  g[12] = -0.125 * v * wp1; g[13] = -0.125 * u * wp1; g[14] = 0.125 * u * v;
+  g[15] = 0.125 * v * wp1;  g[16] = -0.125 * up1 * wp1; g[17] = 0.125 * up1 * v;
+  g[18] = 0.125 * vp1 * wp1; g[19] = 0.125 * up1 * wp1; g[20] = 0.125 * up1 * vp1;
+  g[21] = -0.125 * vp1 * wp1; g[22] = 0.125 * u * wp1; g[23] = 0.125 * u * vp1;
+}
+
+void orc_compute_gradient_values(double gv[192]) {
+  double gp[2], gw[2], pt[3];
+  orc_gauss_pts(gp, gw);
+  double* p = gv;
+  for (int ig = 0; ig < 2; ++ig) {
+    pt[0] = gp[ig];
+    for (int jg = 0; jg < 2; ++jg) {
+      pt[1] = gp[jg];
+      for (int kg = 0; kg < 2; ++kg) {
+        pt[2] = gp[kg];
+        orc_gradients(pt, p);
+        p += 24;
+      }
+    }
+  }
+}
+
+/* Hex8.hpp:115-175 / :177-257: J = sum_i grad_i (x) x_i, accumulated node by
+ * node; returns the cofactor terms so callers share one evaluation order. */
+static void jacobian(const double* c, const double* gv, double J[9]) {
+  double J00 = 0, J01 = 0, J02 = 0, J10 = 0, J11 = 0, J12 = 0, J20 = 0, J21 = 0, J22 = 0;
+  for (int i = 0; i < 8; ++i) {
+    const double* g = gv + 3 * i;
+    const double* x = c + 3 * i;
+    J00 += g[0] * x[0]; J01 += g[0] * x[1]; J02 += g[0] * x[2];
+    J10 += g[1] * x[0]; J11 += g[1] * x[1]; J12 += g[1] * x[2];
+    J20 += g[2] * x[0]; J21 += g[2] * x[1]; J22 += g[2] * x[2];
+  }
+  J[0] = J00; J[1] = J01; J[2] = J02;
+  J[3] = J10; J[4] = J11; J[5] = J12;
+  J[6] = J20; J[7] = J21; J[8] = J22;
+}
+
+static double det_from(const double J[9], double* t0, double* t1, double* t2) {
+  const double J00 = J[0], J01 = J[1], J02 = J[2], J10 = J[3], J11 = J[4], J12 = J[5],
+               J20 = J[6], J21 = J[7], J22 = J[8];
+  *t0 = J22 * J11 - J21 * J12;
+  *t1 = J22 * J01 - J21 * J02;
+  *t2 = J12 * J01 - J11 * J02;
+  return J00 * *t0 - J10 * *t1 + J20 * *t2;
+}
+
+void orc_diffusion_matrix_symm(const double* coords, const double* grad_vals, double k,
+                               double* elem_mat) {
+  double gpts[2], gwts[2];
+  for (int i = 0; i < 36; ++i) elem_mat[i] = 0;
+  orc_gauss_pts(gpts, gwts);
+  double dpsidx[8], dpsidy[8], dpsidz[8], invJ[9], J[9];
+  size_t gv_offset = 0;
+  for (int ig = 0; ig < 2; ++ig) {
+    double wi = gwts[ig];
+    for (int jg = 0; jg < 2; ++jg) {
+      double wi_wj = wi * gwts[jg];
+      for (int kg = 0; kg < 2; ++kg) {
+        double wi_wj_wk = wi_wj * gwts[kg];
+        const double* gvp = &grad_vals[gv_offset];
+        gv_offset += 24;
+        /* gradients_and_invJ_and_detJ, Hex8.hpp:177-257 */
+        jacobian(coords, gvp, J);
+        double term0, term1, term2;
+        double detJ = det_from(J, &term0, &term1, &term2);
+        const double J00 = J[0], J01 = J[1], J02 = J[2], J10 = J[3], J11 = J[4],
+                     J12 = J[5], J20 = J[6], J21 = J[7], J22 = J[8];
+        double inv_detJ = 1.0 / detJ;
+        invJ[0] = term0 * inv_detJ;
+        invJ[1] = -term1 * inv_detJ;
+        invJ[2] = term2 * inv_detJ;
+        invJ[3] = -(J22 * J10 - J20 * J12) * inv_detJ;
+        invJ[4] = (J22 * J00 - J20 * J02) * inv_detJ;
+        invJ[5] = -(J12 * J00 - J10 * J02) * inv_detJ;
+        invJ[6] = (J21 * J10 - J20 * J11) * inv_detJ;
+        invJ[7] = -(J21 * J00 - J20 * J01) * inv_detJ;
+        invJ[8] = (J11 * J00 - J10 * J01) * inv_detJ;
+
+        double k_detJ_wi_wj_wk = k * detJ * wi_wj_wk;
+        const double* g = gvp;
+        for (int i = 0; i < 8; ++i) {
+          double g0 = g[0], g1 = g[1], g2 = g[2];
+          dpsidx[i] = g0 * invJ[0] + g1 * invJ[1] + g2 * invJ[2];
+          dpsidy[i] = g0 * invJ[3] + g1 * invJ[4] + g2 * invJ[5];
+          dpsidz[i] = g0 * invJ[6] + g1 * invJ[7] + g2 * invJ[8];
+          g += 3;
+        }
+        int offset = 0;
+        for (int m = 0; m < 8; ++m) {
+          const double dxm = dpsidx[m], dym = dpsidy[m], dzm = dpsidz[m];
+          elem_mat[offset++] += k_detJ_wi_wj_wk * ((dxm * dxm) + (dym * dym) + (dzm * dzm));
+          for (int n = m + 1; n < 8; ++n)
+            elem_mat[offset++] +=
+                k_detJ_wi_wj_wk * ((dxm * dpsidx[n]) + (dym * dpsidy[n]) + (dzm * dpsidz[n]));
+        }
+      }
+    }
+  }
+}
+
+void orc_source_vector(const double* coords, const double* grad_vals, double Q,
+                       double* elem_vec) {
+  double gpts[2], gwts[2], psi[8], pt[3], J[9];
+  for (int i = 0; i < 8; ++i) elem_vec[i] = 0;
+  orc_gauss_pts(gpts, gwts);
+  size_t gv_offset = 0;
+  for (int ig = 0; ig < 2; ++ig) {
+    pt[0] = gpts[ig];
+    double wi = gwts[ig];
+    for (int jg = 0; jg < 2; ++jg) {
+      pt[1] = gpts[jg];
+      double wj = gwts[jg];
+      for (int kg = 0; kg < 2; ++kg) {
+        pt[2] = gpts[kg];
+        double wk = gwts[kg];
+        orc_shape_fns(pt, psi);
+        const double* gvp = &grad_vals[gv_offset];
+        gv_offset += 24;
+        /* gradients_and_detJ, Hex8.hpp:115-175 */
+        jacobian(coords, gvp, J);
+        double t0, t1, t2;
+        double detJ = det_from(J, &t0, &t1, &t2);
+        double term = Q * detJ * wi * wj * wk;
+        for (int i = 0; i < 8; ++i) elem_vec[i] += psi[i] * term;
+      }
+    }
+  }
+}
+
+/* Hex8_box_utils.hpp:41-57 (exodus order), :59-99 (x0 + h, not (i+1)/N),
+ * :101-159 (node ids mapped to rows: identity on one rank). */
+void orc_get_elem_nodes_and_coords(const int gbox[6], int elem_id, int* ids,
+                                   double* c) {
+  int gx = gbox[1] + 1, gy = gbox[3] + 1, gz = gbox[5] + 1;
+  int ex, ey, ez;
+  get_int_coords(elem_id, gx - 1, gy - 1, &ex, &ey, &ez);
+  int node0 = get_id(gx, gy, gz, ex, ey, ez);
+  ids[0] = node0;
+  ids[1] = node0 + 1;
+  ids[2] = node0 + gx + 1;
+  ids[3] = node0 + gx;
+  ids[4] = node0 + gx * gy;
+  ids[5] = node0 + 1 + gx * gy;
+  ids[6] = node0 + gx + gx * gy + 1;
+  ids[7] = node0 + gx + gx * gy;
+  double x, y, z;
+  get_coords(node0, gx, gy, gz, &x, &y, &z);
+  double hx = 1.0 / gbox[1], hy = 1.0 / gbox[3], hz = 1.0 / gbox[5];
+  c[0] = x;       c[1] = y;       c[2] = z;
+  c[3] = x + hx;  c[4] = y;       c[5] = z;
+  c[6] = x + hx;  c[7] = y + hy;  c[8] = z;
+  c[9] = x;       c[10] = y + hy; c[11] = z;
+  c[12] = x;      c[13] = y;      c[14] = z + hz;
+  c[15] = x + hx; c[16] = y;      c[17] = z + hz;
+  c[18] = x + hx; c[19] = y + hy; c[20] = z + hz;
+  c[21] = x;      c[22] = y + hy; c[23] = z + hz;
+}
+
+/* SparseMatrix_functions.hpp:142-160: lower_bound in the sorted row, add if found. */
+static void sum_into_row(int row_len, const int* row_idx, double* row_coefs, int num,
+                         const int* in_idx, const double* in_coefs) {
+  for (int i = 0; i < num; ++i) {
+    int lo = 0, hi = row_len;
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (row_idx[mid] < in_idx[i]) lo = mid + 1; else hi = mid;
+    }
+    if (lo < row_len && row_idx[lo] == in_idx[i]) row_coefs[lo] += in_coefs[i];
+  }
+}
+
+int orc_perform_element_loop(const int gbox[6], const int eb[6], int64_t nrows,
+                             const int* row_offsets, const int* cols, double* vals,
+                             double* b, const double* kappa) {
+  double gv[192], coords[24], K[36], f[8];
+  int ids[8];
+  orc_compute_gradient_values(gv);
+  const int gex = gbox[1], gey = gbox[3], gez = gbox[5];
+  /* BoxIterator order (BoxIterator.hpp:79-96): x fastest -> ascending id. */
+  for (int z = eb[4]; z < eb[5]; ++z)
+    for (int y = eb[2]; y < eb[3]; ++y)
+      for (int x = eb[0]; x < eb[1]; ++x) {
+        int e = get_id(gex, gey, gez, x, y, z);
+        orc_get_elem_nodes_and_coords(gbox, e, ids, coords);
+        orc_diffusion_matrix_symm(coords, gv, kappa ? kappa[e] : 1.0, K);
+        orc_source_vector(coords, gv, 1.0, f);
+        /* sum_in_symm_elem_matrix, SparseMatrix_functions.hpp:183-227 */
+        int row_offset = 0;
+        for (int i = 0; i < 8; ++i) {
+          int row = ids[i];
+          const double* row_coefs = &K[row_offset];
+          int row_len = 8 - i;
+          row_offset += row_len;
+          if (row < 0 || row >= nrows) continue;
+          int o = row_offsets[row], len = row_offsets[row + 1] - o;
+          if (len == 0) continue;
+          sum_into_row(len, cols + o, vals + o, row_len, &ids[i], row_coefs);
+          int offset = i;
+          for (int j = 0; j < i; ++j) {
+            double coef = K[offset];
+            sum_into_row(len, cols + o, vals + o, 1, &ids[j], &coef);
+            offset += 8 - (j + 1);
+          }
+        }
+        /* sum_into_vector, Vector_functions.hpp:84-103 */
+        for (int i = 0; i < 8; ++i)
+          if (ids[i] >= 0 && ids[i] < nrows) b[ids[i]] += f[i];
+      }
+  return 0;
+}
+
+int orc_assemble_FE_data(const int gbox[6], int64_t nrows, const int* row_offsets,
+                         const int* cols, double* vals, double* b, const double* kappa) {
+  if (!box_is_origin(gbox)) return 1;
+  /* One rank: local box == global box, so the ghost layer adds nothing
+   * (assemble_FE_data.hpp:59-72). */
+  return orc_perform_element_loop(gbox, gbox, nrows, row_offsets, cols, vals, b, kappa);
+}
+
+int64_t orc_bc_rows(const int gbox[6], int which, int* out) {
+  const int gx = gbox[1] + 1, gy = gbox[3] + 1, gz = gbox[5] + 1;
+  int64_t cnt = 0;
+  for (int iz = 0; iz < gz; ++iz)
+    for (int iy = 0; iy < gy; ++iy)
+      for (int ix = 0; ix < gx; ++ix) {
+        int in0 = ix == 0 || iy == 0 || iz == 0 || iy == gy - 1 || iz == gz - 1;
+        int in1 = ix == gx - 1;
+        if (which ? in1 : in0) {
+          if (out) out[cnt] = get_id(gx, gy, gz, ix, iy, iz);
+          ++cnt;
+        }
+      }
+  return cnt;
+}
+
+static int in_set(const int* s, int64_t n, int v) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (s[mid] < v) lo = mid + 1; else hi = mid;
+  }
+  return lo < n && s[lo] == v;
+}
+
+void orc_impose_dirichlet(double value, int64_t nrows, const int* row_offsets,
+                          const int* cols, double* vals, double* b, const int* bc,
+                          int64_t nbc) {
+  const int first = 0, last = (int)nrows - 1;
+  for (int64_t k = 0; k < nbc; ++k) {
+    int row = bc[k];
+    if (row >= first && row <= last) {
+      b[row] = value;
+      for (int j = row_offsets[row]; j < row_offsets[row + 1]; ++j)
+        vals[j] = (cols[j] == row) ? 1 : 0;
+    }
+  }
+  for (int64_t i = 0; i < nrows; ++i) {
+    if (in_set(bc, nbc, (int)i)) continue;
+    double sum = 0;
+    for (int j = row_offsets[i]; j < row_offsets[i + 1]; ++j)
+      if (in_set(bc, nbc, cols[j])) {
+        sum += vals[j];
+        vals[j] = 0;
+      }
+    b[i] -= sum * value;
+  }
+}
+
+void orc_matvec(int64_t n, const int* ro, const int* cols, const double* vals,
+                const double* x, double* y) {
+  const double beta = 0;
+  for (int64_t row = 0; row < n; ++row) {
+    double sum = beta * y[row];
+    for (int i = ro[row]; i < ro[row + 1]; ++i) sum += vals[i] * x[cols[i]];
+    y[row] = sum;
+  }
+}
+
+double orc_dot(int64_t n, const double* x, const double* y) {
+  double r = 0;
+  for (int64_t i = 0; i < n; ++i) r += x[i] * y[i];
+  return r;
+}
+
+void orc_waxpby(int64_t n, double alpha, const double* x, double beta, const double* y,
+                double* w) {
+  for (int64_t i = 0; i < n; ++i) w[i] = alpha * x[i] + beta * y[i];
+}
+
+/* cg_solve.hpp:49-67 */
+static int breakdown(int64_t n, double inner, const double* v, const double* w) {
+  double vnorm = sqrt(orc_dot(n, v, v));
+  double wnorm = sqrt(orc_dot(n, w, w));
+  return fabs(inner) <= 100 * vnorm * wnorm * DBL_EPSILON;
+}
+
+int orc_cg_solve(int64_t n, const int* ro, const int* cols, const double* vals,
+                 const double* b, double* x, int max_iter, double tolerance,
+                 int* num_iters, double* normr_out, double* history, double* times) {
+  double tW = 0, tD = 0, tM = 0, t0;
+  double total = now_s();
+  double* r = calloc((size_t)n, sizeof(double));
+  double* p = calloc((size_t)n, sizeof(double));
+  double* Ap = calloc((size_t)n, sizeof(double));
+  if (!r || !p || !Ap) { free(r); free(p); free(Ap); return 2; }
+  double normr = 0, rtrans = 0, oldrtrans = 0;
+  const double one = 1.0, zero = 0.0;
+  int status = 0;
+
+  t0 = now_s(); orc_waxpby(n, one, x, zero, x, p); tW += now_s() - t0;
+  t0 = now_s(); orc_matvec(n, ro, cols, vals, p, Ap); tM += now_s() - t0;
+  t0 = now_s(); orc_waxpby(n, one, b, -one, Ap, r); tW += now_s() - t0;
+  t0 = now_s(); rtrans = orc_dot(n, r, r); tD += now_s() - t0;
+  normr = sqrt(rtrans);
+  double brkdown_tol = DBL_EPSILON;
+
+  for (int k = 1; k <= max_iter && normr > tolerance; ++k) {
+    double beta = 0;
+    if (k == 1) {
+      t0 = now_s(); orc_waxpby(n, one, r, zero, r, p); tW += now_s() - t0;
+    } else {
+      oldrtrans = rtrans;
+      t0 = now_s(); rtrans = orc_dot(n, r, r); tD += now_s() - t0;
+      beta = rtrans / oldrtrans;
+      t0 = now_s(); orc_waxpby(n, one, r, beta, p, p); tW += now_s() - t0;
+    }
+    normr = sqrt(rtrans);
+    t0 = now_s(); orc_matvec(n, ro, cols, vals, p, Ap); tM += now_s() - t0;
+    t0 = now_s(); double p_ap_dot = orc_dot(n, Ap, p); tD += now_s() - t0;
+    if (history) {
+      history[4 * (k - 1) + 0] = rtrans;
+      history[4 * (k - 1) + 1] = p_ap_dot;
+      history[4 * (k - 1) + 2] = 0;
+      history[4 * (k - 1) + 3] = beta;
+    }
+    if (p_ap_dot < brkdown_tol) {
+      if (p_ap_dot < 0 || breakdown(n, p_ap_dot, Ap, p)) { status = 1; break; }
+      brkdown_tol = 0.1 * p_ap_dot;
+    }
+    double alpha = rtrans / p_ap_dot;
+    if (history) history[4 * (k - 1) + 2] = alpha;
+    t0 = now_s();
+    orc_waxpby(n, one, x, alpha, p, x);
+    orc_waxpby(n, one, r, -alpha, Ap, r);
+    tW += now_s() - t0;
+    *num_iters = k;
+  }
+  if (times) {
+    times[0] = tW; times[1] = tD; times[2] = tM; times[3] = 0; times[4] = now_s() - total;
+  }
+  *normr_out = normr;
+  free(r); free(p); free(Ap);
+  return status;
+}
+
+/* analytic_soln.hpp:51-112 */
+static const double PI = 3.141592653589793238462;
+double orc_soln(double x, double y, double z, int max_p, int max_q) {
+  const double PI_SQR = PI * PI;
+  const double term0 = 16.0 / (PI_SQR);
+  double sum = 0;
+  for (int p = 0; p <= max_p; ++p) {
+    const double p21y = (2 * p + 1) * PI * y;
+    const double sin_py = sin(p21y) / (2 * p + 1);
+    for (int q = 0; q <= max_q; ++q) {
+      const double q21z = (2 * q + 1) * PI * z;
+      const double sin_qz = sin(q21z) / (2 * q + 1);
+      const double l = sqrt((2 * p + 1) * (2 * p + 1) * PI_SQR + (2 * q + 1) * (2 * q + 1) * PI_SQR);
+      const double sinh1 = sinh(l * x);
+      const double sinh2 = sinh(l);
+      const double tmp = (sinh1 * sin_py) * (sin_qz / sinh2);
+      if (tmp == tmp) sum += tmp; else break;
+    }
+  }
+  return term0 * sum;
+}
+
+int orc_verify_solution(const int gbox[6], const double* xv, double tolerance,
+                        int whole, double* max_err) {
+  const int gx = gbox[1] + 1, gy = gbox[3] + 1, gz = gbox[5] + 1;
+  double maxe = 0.0;
+  int roffset = 0;
+  for (int iz = 0; iz < gz; ++iz)
+    for (int iy = 0; iy < gy; ++iy)
+      for (int ix = 0; ix < gx; ++ix) {
+        int id = get_id(gx, gy, gz, ix, iy, iz);
+        double x, y, z;
+        get_coords(id, gx, gy, gz, &x, &y, &z);
+        int check = whole || (fabs(x - 0.5) < 0.05 && fabs(y - 0.5) < 0.05 &&
+                              fabs(z - 0.5) < 0.05);
+        if (check) {
+          double a;
+          if (x == 1.0) a = 1;
+          else if (x == 0.0 || y == 0.0 || z == 0.0) a = 0;
+          else if (y == 1.0 || z == 1.0) a = 0;
+          else a = orc_soln(x, y, z, 300, 300);
+          double err = fabs(a - xv[roffset]);
+          if (err > maxe) maxe = err;
+        }
+        ++roffset;
+      }
+  if (max_err) *max_err = maxe;
+  return maxe > tolerance ? 1 : 0;
+}

@@ -1,0 +1,55 @@
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+SMALL_MESHES = [(1, 1, 1), (2, 2, 2), (1, 1, 2), (2, 1, 1), (3, 4, 5), (17, 3, 5), (10, 10, 10)]
+MEDIUM_MESHES = [(20, 30, 40), (30, 30, 30)]
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
+    config.addinivalue_line("markers", "slow: long-running (large meshes)")
+
+
+def golden_npz(dims):
+    nx, ny, nz = dims
+    path = os.path.join(GOLDEN, f"ref_mesh_{nx}x{ny}x{nz}.npz")
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def golden_json(dims):
+    nx, ny, nz = dims
+    path = os.path.join(GOLDEN, f"ref_mesh_{nx}x{ny}x{nz}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def golden_elem():
+    with np.load(os.path.join(GOLDEN, "ref_elem.npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def bits_equal(a, b) -> bool:
+    """Bit-for-bit equality of two float64 arrays (also distinguishes -0.0)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    return a.shape == b.shape and bool(np.array_equal(a.view(np.uint64), b.view(np.uint64)))
+
+
+@pytest.fixture(scope="session")
+def oracle_mod():
+    import oracle
+    if not os.path.exists(oracle.LIB_PATH):
+        oracle.build()
+    return oracle

@@ -1,0 +1,172 @@
+"""Generate the golden fixtures from the compiled reference (oracle/_ref).
+
+Runs only where the reference source tree exists (it builds oracle/_ref with
+oracle/Makefile).  Writes small, committed fixtures into tests/golden/:
+
+  ref_mesh_<nx>x<ny>x<nz>.npz   full arrays for small meshes: structure,
+                                assembled A/b before BCs, after the u=0 pass,
+                                after the u=1 pass, bc sets, CG solution and
+                                per-iteration (pAp, rtrans, alpha) history
+  ref_mesh_<...>.json           large meshes: SHA-256 of every array, sampled
+                                rows, CG history, final x digest
+  ref_elem.npz                  Hex8 element matrix / source vector cases
+
+Usage:  python tests/golden/make_golden.py [--big]   (--big adds 300^3, ~3 min)
+"""
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+import oracle  # noqa: E402
+
+SMALL = [(1, 1, 1), (2, 2, 2), (1, 1, 2), (2, 1, 1), (3, 4, 5), (17, 3, 5), (10, 10, 10)]
+MEDIUM = [(20, 30, 40), (30, 30, 30)]
+BIG = [(300, 300, 300)]
+ARRAYS = {
+    "A_prebc_rows": np.int32, "A_prebc_row_offsets": np.int32, "A_prebc_cols": np.int32,
+    "A_prebc_vals": np.float64, "b_prebc": np.float64,
+    "A_bc0_vals": np.float64, "b_bc0": np.float64,
+    "A_vals": np.float64, "A_cols": np.int32, "A_row_offsets": np.int32, "b": np.float64,
+    "bc_rows_0": np.int32, "bc_rows_1": np.int32, "x": np.float64,
+}
+
+
+def run_ref(nx, ny, nz, max_iters, workdir):
+    subprocess.run([oracle.REF_DUMP, "mesh", str(nx), str(ny), str(nz), workdir,
+                    str(max_iters)], check=True, stdout=subprocess.DEVNULL)
+    meta = json.load(open(os.path.join(workdir, "meta.json")))
+    hist = []
+    meta["breakdown"] = 0
+    for line in open(os.path.join(workdir, "minife_debug.1.0")):
+        if not line.startswith("iter"):
+            continue
+        # "iter k, p_ap_dot = X, rtrans = Y, alpha = Z" (cg_solve.hpp:182-204), or
+        # "iter k, p_ap_dot = XERROR, numerical breakdown!" when cg_solve stops.
+        pap = line.split("p_ap_dot = ")[1]
+        if "ERROR" in pap:
+            hist.append([float(pap.split("ERROR")[0]), np.nan, np.nan])
+            meta["breakdown"] = 1
+            continue
+        parts = line.replace(",", "").split()
+        hist.append([float(parts[4]), float(parts[7]), float(parts[10])])
+    return meta, np.array(hist, dtype=np.float64).reshape(-1, 3)
+
+
+def load(workdir, name):
+    return np.fromfile(os.path.join(workdir, name + ".bin"), dtype=ARRAYS[name])
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def sample_rows(nrows: int) -> np.ndarray:
+    rng = np.random.default_rng(0)
+    pick = np.concatenate([np.arange(min(40, nrows)), np.arange(max(0, nrows - 40), nrows),
+                           rng.integers(0, nrows, 120)])
+    return np.unique(pick).astype(np.int64)
+
+
+def small_case(dims, tmp):
+    nx, ny, nz = dims
+    meta, hist = run_ref(nx, ny, nz, 200, tmp)
+    out = {k: load(tmp, k) for k in ARRAYS}
+    out["history"] = hist
+    out["meta"] = np.array([meta["num_iters"], meta["verify_rc"], meta["breakdown"]],
+                           dtype=np.int64)
+    out["normr"] = np.array([meta["normr_dec"]], dtype=np.float64)
+    path = os.path.join(HERE, f"ref_mesh_{nx}x{ny}x{nz}.npz")
+    np.savez_compressed(path, **out)
+    return path
+
+
+def large_case(dims, tmp, max_iters):
+    nx, ny, nz = dims
+    meta, hist = run_ref(nx, ny, nz, max_iters, tmp)
+    rec = dict(nx=nx, ny=ny, nz=nz, max_iters=max_iters, num_iters=meta["num_iters"],
+               normr=meta["normr_dec"], verify_rc=meta["verify_rc"], breakdown=meta["breakdown"],
+               nrows=meta["nrows"], nnz=meta["nnz"], sha256={}, history=hist.tolist())
+    ro = load(tmp, "A_row_offsets")
+    rows = sample_rows(meta["nrows"])
+    samp = {"rows": rows.tolist()}
+    for k in ARRAYS:
+        a = load(tmp, k)
+        rec["sha256"][k] = sha(a)
+        if k in ("b_prebc", "b", "x", "b_bc0"):
+            samp[k] = [float(v).hex() for v in a[rows]]
+        del a
+    for k in ("A_prebc_vals", "A_vals", "A_cols"):
+        a = load(tmp, k)
+        samp[k] = [[(float(v).hex() if a.dtype == np.float64 else int(v))
+                    for v in a[ro[r]:ro[r + 1]]] for r in rows]
+        del a
+    rec["samples"] = samp
+    path = os.path.join(HERE, f"ref_mesh_{nx}x{ny}x{nz}.json")
+    with open(path, "w") as f:
+        json.dump(rec, f, indent=0)
+    return path
+
+
+def elem_cases(tmp):
+    """Element coordinates: mesh elements exactly as get_elem_nodes_and_coords
+    makes them, the unit cube, and seeded distorted hexes."""
+    coords = []
+    for (nx, ny, nz), elems in [((1, 1, 1), [0]), ((30, 30, 30), [0, 1, 931, 13456, 26999]),
+                                ((300, 300, 300), [0, 4_500_150, 13_545_149, 26_999_999]),
+                                ((128, 128, 128), [0, 999_999, 2_097_151]),
+                                ((17, 3, 5), [0, 100, 254])]:
+        for e in elems:
+            coords.append(oracle.elem_nodes_and_coords(nx, ny, nz, e)[1])
+    rng = np.random.default_rng(0)
+    _, unit = oracle.elem_nodes_and_coords(1, 1, 1, 0)
+    for _ in range(24):
+        coords.append(unit * rng.uniform(0.2, 3.0) + rng.uniform(-0.12, 0.12, 24))
+    coords = np.array(coords, dtype=np.float64)
+    fin, fout = os.path.join(tmp, "elem_in.bin"), os.path.join(tmp, "elem_out.bin")
+    coords.tofile(fin)
+    subprocess.run([oracle.REF_DUMP, "elem", fin, fout], check=True)
+    res = np.fromfile(fout, dtype=np.float64).reshape(-1, 44)
+    path = os.path.join(HERE, "ref_elem.npz")
+    np.savez_compressed(path, coords=coords, K=res[:, :36], f=res[:, 36:])
+    return path
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--big", action="store_true", help="also run 300^3 (minutes, ~30 GB disk)")
+    ap.add_argument("--big-iters", type=int, default=12)
+    args = ap.parse_args()
+    if not os.path.isdir("/root/reference/ref"):
+        sys.exit("the reference tree is not present here; fixtures are generated where it is")
+    oracle.build()
+    for dims in SMALL:
+        with tempfile.TemporaryDirectory() as t:
+            print("wrote", small_case(dims, t))
+    for dims in MEDIUM:
+        with tempfile.TemporaryDirectory() as t:
+            print("wrote", large_case(dims, t, 200))
+    with tempfile.TemporaryDirectory() as t:
+        print("wrote", elem_cases(t))
+    if args.big:
+        for dims in BIG:
+            t = tempfile.mkdtemp(dir="/root/repo/_work")
+            try:
+                print("wrote", large_case(dims, t, args.big_iters))
+            finally:
+                shutil.rmtree(t)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,327 @@
+"""Pins the CPU oracle (oracle/minife_oracle.c) before it is trusted.
+
+Three kinds of evidence, per function:
+  * golden vectors captured from the compiled reference (tests/golden/*),
+    compared bit for bit;
+  * the reference's own known answers (ref/basic/utest_cases.hpp);
+  * what mathematics fixes independently of both: closed forms, exact
+    rational evaluation of the element integrals, brute-force structure, and
+    invariants of the assembled operator and of CG.
+"""
+import hashlib
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from conftest import (MEDIUM_MESHES, SMALL_MESHES, bits_equal, golden_elem, golden_json,
+                      golden_npz)
+
+EPS = float(np.finfo(np.float64).eps)
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+# --------------------------------------------------------------- structure --
+
+@pytest.mark.parametrize("dims", SMALL_MESHES)
+def test_structure_matches_reference(oracle_mod, dims):
+    g = golden_npz(dims)
+    A = oracle_mod.generate_matrix_structure(*dims)
+    assert np.array_equal(A.rows, g["A_prebc_rows"])
+    assert np.array_equal(A.row_offsets, g["A_prebc_row_offsets"])
+    assert np.array_equal(A.cols, g["A_prebc_cols"])
+
+
+def test_structure_known_answers(oracle_mod):
+    # utest_cases.hpp:130-156 and :158-192
+    assert oracle_mod.generate_matrix_structure(1, 1, 1).nnz == 64
+    A = oracle_mod.generate_matrix_structure(2, 2, 2)
+    assert A.nrows == 27 and A.nnz == 343
+    assert A.row_offsets[14] - A.row_offsets[13] == 27
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (3, 4, 5), (4, 2, 3)])
+def test_structure_brute_force(oracle_mod, dims):
+    nx, ny, nz = dims
+    gx, gy, gz = nx + 1, ny + 1, nz + 1
+    A = oracle_mod.generate_matrix_structure(*dims)
+    assert A.nnz == (3 * gx - 2) * (3 * gy - 2) * (3 * gz - 2)
+    for r in range(A.nrows):
+        ix, iy, iz = r % gx, (r // gx) % gy, r // (gx * gy)
+        want = sorted(x + gx * y + gx * gy * z
+                      for z in range(iz - 1, iz + 2) for y in range(iy - 1, iy + 2)
+                      for x in range(ix - 1, ix + 2)
+                      if 0 <= x < gx and 0 <= y < gy and 0 <= z < gz)
+        assert list(A.cols[A.row_offsets[r]:A.row_offsets[r + 1]]) == want
+
+
+# ----------------------------------------------------------------- element --
+
+def test_element_matches_reference(oracle_mod):
+    g = golden_elem()
+    for c, K, f in zip(g["coords"], g["K"], g["f"]):
+        k2, f2 = oracle_mod.element(c)
+        assert bits_equal(k2, K)
+        assert bits_equal(f2, f)
+
+
+A_GAUSS = Fraction("0.577350269")  # gauss_pts.hpp:41
+
+
+def _exact_element(coords):
+    """K_e and f_e in exact rational arithmetic from the textbook definition:
+    K_ij = sum_gp w detJ (J^-1 grad psi_i).(J^-1 grad psi_j), f_i = sum_gp w detJ psi_i,
+    with J_rc = sum_n dpsi_n/dxi_r * x_n,c and the truncated 2-point abscissa."""
+    X = [[Fraction(float(coords[3 * n + d])) for d in range(3)] for n in range(8)]
+    corner = [(-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1),
+              (-1, -1, 1), (1, -1, 1), (1, 1, 1), (-1, 1, 1)]
+    K = [[Fraction(0)] * 8 for _ in range(8)]
+    f = [Fraction(0)] * 8
+    for s0 in (-1, 1):
+        for s1 in (-1, 1):
+            for s2 in (-1, 1):
+                xi = (s0 * A_GAUSS, s1 * A_GAUSS, s2 * A_GAUSS)
+                psi, dpsi = [], []
+                for (a, b, c) in corner:
+                    fa, fb, fc = 1 + a * xi[0], 1 + b * xi[1], 1 + c * xi[2]
+                    psi.append(fa * fb * fc / 8)
+                    dpsi.append((a * fb * fc / 8, fa * b * fc / 8, fa * fb * c / 8))
+                J = [[sum(dpsi[n][r] * X[n][c] for n in range(8)) for c in range(3)]
+                     for r in range(3)]
+                det = (J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1])
+                       - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0])
+                       + J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]))
+                adj = [[J[1][1] * J[2][2] - J[1][2] * J[2][1], J[0][2] * J[2][1] - J[0][1] * J[2][2],
+                        J[0][1] * J[1][2] - J[0][2] * J[1][1]],
+                       [J[1][2] * J[2][0] - J[1][0] * J[2][2], J[0][0] * J[2][2] - J[0][2] * J[2][0],
+                        J[0][2] * J[1][0] - J[0][0] * J[1][2]],
+                       [J[1][0] * J[2][1] - J[1][1] * J[2][0], J[0][1] * J[2][0] - J[0][0] * J[2][1],
+                        J[0][0] * J[1][1] - J[0][1] * J[1][0]]]
+                g = [[sum(adj[d][r] * dpsi[n][r] for r in range(3)) / det for d in range(3)]
+                     for n in range(8)]
+                for i in range(8):
+                    f[i] += psi[i] * det
+                    for j in range(8):
+                        K[i][j] += det * sum(g[i][d] * g[j][d] for d in range(3))
+    return K, f
+
+
+def _unpack(Kp):
+    M = np.zeros((8, 8))
+    o = 0
+    for m in range(8):
+        for n in range(m, 8):
+            M[m, n] = M[n, m] = Kp[o]
+            o += 1
+    return M
+
+
+@pytest.mark.parametrize("case", [0, 1, 7, 12, 20, 30, 39])
+def test_element_exact_rational(oracle_mod, case):
+    coords = golden_elem()["coords"][case]
+    Kx, fx = _exact_element(coords)
+    Kf, ff = oracle_mod.element(coords)
+    M = _unpack(Kf)
+    # J = sum g_n x_n cancels when the element is small and far from the
+    # origin: the rounding error scales with |x| / edge length.
+    xyz = np.asarray(coords).reshape(8, 3)
+    edge = min(np.linalg.norm(xyz[1] - xyz[0]), np.linalg.norm(xyz[3] - xyz[0]),
+               np.linalg.norm(xyz[4] - xyz[0]))
+    cond = 1.0 + np.abs(xyz).max() / edge
+    tol = 64 * EPS * cond
+    scale = max(abs(float(v)) for row in Kx for v in row)
+    for i in range(8):
+        for j in range(8):
+            assert abs(M[i, j] - float(Kx[i][j])) <= tol * scale
+        assert abs(ff[i] - float(fx[i])) <= tol * max(abs(float(v)) for v in fx)
+
+
+def test_element_closed_forms(oracle_mod):
+    # Unit cube with abscissa a: diagonal -> 1/3, face diagonal and body
+    # diagonal -> -1/12, edge neighbour -> (1 - 3a^2)/... ~5.47e-11 (0 for a^2 = 1/3).
+    _, c = oracle_mod.elem_nodes_and_coords(1, 1, 1, 0)
+    M = _unpack(oracle_mod.element(c)[0])
+    Kx, _ = _exact_element(c)
+    assert abs(M[0, 0] - 1 / 3) < 1e-9 and abs(M[0, 2] + 1 / 12) < 1e-9 and abs(M[0, 6] + 1 / 12) < 1e-9
+    # the pseudo-zero is a difference of O(1) terms: exact up to O(eps) absolute
+    assert 5e-11 < M[0, 1] < 6e-11 and abs(M[0, 1] - float(Kx[0][1])) < 64 * EPS / 3
+    # symmetric, zero row sums (grad of a constant vanishes), source = volume/8 per node
+    assert np.allclose(M, M.T, rtol=0, atol=0)
+    assert np.max(np.abs(M.sum(axis=1))) < 1e-15
+    for dims, e in [((30, 30, 30), 13456), ((7, 3, 5), 11)]:
+        _, c = oracle_mod.elem_nodes_and_coords(*dims, e)
+        f = oracle_mod.element(c)[1]
+        h3 = (1 / dims[0]) * (1 / dims[1]) * (1 / dims[2])
+        assert np.allclose(f, h3 / 8, rtol=1e-13, atol=0)
+
+
+def test_kappa_scales_linearly(oracle_mod):
+    _, c = oracle_mod.elem_nodes_and_coords(9, 9, 9, 100)
+    K1 = oracle_mod.element(c, 1.0)[0]
+    K3 = oracle_mod.element(c, 3.0)[0]
+    assert np.allclose(K3, 3 * K1, rtol=1e-15, atol=4 * EPS * np.abs(K3).max())
+
+
+# ---------------------------------------------------------------- assembly --
+
+@pytest.mark.parametrize("dims", SMALL_MESHES)
+def test_assembly_and_dirichlet_match_reference(oracle_mod, dims):
+    g = golden_npz(dims)
+    P = oracle_mod.build_problem(*dims)
+    assert bits_equal(P["A_prebc"].vals, g["A_prebc_vals"])
+    assert bits_equal(P["b_prebc"], g["b_prebc"])
+    assert bits_equal(P["A"].vals, g["A_vals"])
+    assert bits_equal(P["b"], g["b"])
+    assert np.array_equal(P["A"].cols, g["A_cols"])
+    # the u=0 pass alone
+    A = P["A_prebc"].copy()
+    b = P["b_prebc"].copy()
+    oracle_mod.impose_dirichlet(0.0, A, b, oracle_mod.bc_rows(*dims, 0))
+    assert bits_equal(A.vals, g["A_bc0_vals"]) and bits_equal(b, g["b_bc0"])
+    assert np.array_equal(oracle_mod.bc_rows(*dims, 0), g["bc_rows_0"])
+    assert np.array_equal(oracle_mod.bc_rows(*dims, 1), g["bc_rows_1"])
+
+
+def _dense(A):
+    n = A.nrows
+    M = np.zeros((n, n))
+    for r in range(n):
+        for j in range(A.row_offsets[r], A.row_offsets[r + 1]):
+            M[r, A.cols[j]] += A.vals[j]
+    return M
+
+
+@pytest.mark.parametrize("dims", [(2, 2, 2), (3, 2, 4)])
+def test_assembly_brute_force(oracle_mod, dims):
+    """A = sum_e P_e^T K_e P_e and b = sum_e P_e^T f_e, with Dirichlet rows and
+    the lifting b_i -= sum_{c in BC} A_ic g_c, assembled densely."""
+    nx, ny, nz = dims
+    n = (nx + 1) * (ny + 1) * (nz + 1)
+    M = np.zeros((n, n))
+    rhs = np.zeros(n)
+    for e in range(nx * ny * nz):
+        ids, c = oracle_mod.elem_nodes_and_coords(nx, ny, nz, e)
+        K, f = oracle_mod.element(c)
+        M[np.ix_(ids, ids)] += _unpack(K)
+        rhs[ids] += f
+    P = oracle_mod.build_problem(*dims)
+    assert np.allclose(_dense(P["A_prebc"]), M, rtol=1e-14, atol=1e-15)
+    assert np.allclose(P["b_prebc"], rhs, rtol=1e-14, atol=0)
+    # Two passes as driver.hpp:214-218: u=0 on bc_rows_0 first, then u=1 on
+    # bc_rows_1.  The x=1 face's edge nodes are in both sets: the first pass
+    # already zeroed their columns, so only bc_rows_1 \ bc_rows_0 lifts.
+    bc0, bc1 = oracle_mod.bc_rows(*dims, 0), oracle_mod.bc_rows(*dims, 1)
+    lift = np.setdiff1d(bc1, bc0)
+    bc = np.union1d(bc0, bc1)
+    free = np.setdiff1d(np.arange(n), bc)
+    rhs2 = rhs.copy()
+    rhs2[free] -= M[np.ix_(free, lift)] @ np.ones(lift.shape[0])
+    rhs2[bc0] = 0.0
+    rhs2[bc1] = 1.0
+    M2 = M.copy()
+    M2[:, bc] = 0
+    M2[bc, :] = 0
+    M2[bc, bc] = 1
+    assert np.allclose(_dense(P["A"]), M2, rtol=1e-14, atol=1e-15)
+    assert np.allclose(P["b"], rhs2, rtol=1e-13, atol=1e-16)
+
+
+# -------------------------------------------------------- matvec/dot/waxpby --
+
+def test_matvec_known_answer(oracle_mod):
+    # utest_cases.hpp:927-1033: y = [3, 5, -1, 8]
+    A = oracle_mod.CSR(np.arange(4, dtype=np.int32), np.array([0, 2, 6, 8, 10], np.int32),
+                       np.array([0, 1, 0, 1, 2, 3, 1, 2, 1, 3], np.int32),
+                       np.array([1, 1, 2, 1, -1, 1, -2, 1, 2, 1], np.float64))
+    y = oracle_mod.matvec(A, np.array([1.0, 2.0, 3.0, 4.0]))
+    assert list(y) == [3.0, 5.0, -1.0, 8.0]
+
+
+def test_matvec_dot_waxpby_definitions(oracle_mod):
+    P = oracle_mod.build_problem(6, 5, 4)
+    A = P["A"]
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(A.nrows)
+    y = oracle_mod.matvec(A, x)
+    assert np.allclose(y, _dense(A) @ x, rtol=1e-14, atol=1e-14)
+    # row sums in stored order, left to right
+    r = 17
+    s = 0.0
+    for j in range(A.row_offsets[r], A.row_offsets[r + 1]):
+        s += A.vals[j] * x[A.cols[j]]
+    assert y[r] == s
+    d = oracle_mod.dot(x, y)
+    s = 0.0
+    for a, b in zip(x, y):
+        s += a * b
+    assert d == s
+    assert abs(d - math.fsum(x * y)) <= A.nrows * EPS * float(np.abs(x * y).sum())
+    w = oracle_mod.waxpby(0.5, x, -2.0, y)
+    assert bits_equal(w, 0.5 * x + (-2.0) * y)
+
+
+# ---------------------------------------------------------------------- CG --
+
+@pytest.mark.parametrize("dims", SMALL_MESHES)
+def test_cg_matches_reference(oracle_mod, dims):
+    g = golden_npz(dims)
+    P = oracle_mod.build_problem(*dims)
+    r = oracle_mod.cg_solve(P["A"], P["b"], max_iter=200)
+    num_iters, verify_rc, breakdown = (int(v) for v in g["meta"])
+    assert r["num_iters"] == num_iters
+    assert r["status"] == breakdown
+    assert r["normr"] == g["normr"][0]
+    assert bits_equal(r["x"], g["x"])
+    h = g["history"]  # rows: p_ap_dot, rtrans, alpha (NaN after a breakdown)
+    assert h.shape[0] == num_iters + breakdown
+    ours = r["history"]
+    for k in range(num_iters):
+        assert ours[k, 1] == h[k, 0] and ours[k, 0] == h[k, 1] and ours[k, 2] == h[k, 2]
+    assert oracle_mod.verify_solution(*dims, r["x"])[0] == verify_rc
+
+
+@pytest.mark.parametrize("dims", MEDIUM_MESHES)
+def test_medium_meshes_match_reference(oracle_mod, dims):
+    g = golden_json(dims)
+    P = oracle_mod.build_problem(*dims)
+    assert sha(P["A_prebc"].row_offsets) == g["sha256"]["A_prebc_row_offsets"]
+    assert sha(P["A_prebc"].cols) == g["sha256"]["A_prebc_cols"]
+    assert sha(P["A_prebc"].vals) == g["sha256"]["A_prebc_vals"]
+    assert sha(P["b_prebc"]) == g["sha256"]["b_prebc"]
+    assert sha(P["A"].vals) == g["sha256"]["A_vals"]
+    assert sha(P["b"]) == g["sha256"]["b"]
+    r = oracle_mod.cg_solve(P["A"], P["b"], max_iter=g["max_iters"])
+    assert r["num_iters"] == g["num_iters"] and r["normr"] == g["normr"]
+    assert sha(r["x"]) == g["sha256"]["x"]
+    h = np.array(g["history"])
+    assert np.array_equal(r["history"][:, [1, 0, 2]], h)
+
+
+def test_cg_invariants(oracle_mod):
+    """Converged CG solves A x = b; early residuals are mutually orthogonal."""
+    P = oracle_mod.build_problem(8, 8, 8)
+    A, b = P["A"], P["b"]
+    r = oracle_mod.cg_solve(A, b, max_iter=500)
+    res = b - oracle_mod.matvec(A, r["x"])
+    assert np.linalg.norm(res) <= 1e-12 * np.linalg.norm(b)
+    xs = np.linalg.solve(_dense(A), b)
+    assert np.allclose(r["x"], xs, rtol=0, atol=1e-12)
+    hist = oracle_mod.cg_solve(A, b, max_iter=5)["history"]
+    assert np.all(hist[:, 2] > 0) and np.all(hist[:, 1] > 0)
+
+
+def test_analytic_solution(oracle_mod):
+    # Laplace problem with u=1 on one face of the cube: centre value 1/6 by
+    # symmetry (superposing the six rotated problems gives u = 1).
+    s = oracle_mod.soln
+    assert abs(s(0.5, 0.5, 0.5) - 1 / 6) < 1e-14
+    # The six one-face problems are rotations of this one and sum to u = 1.
+    for (x, y, z) in [(0.3, 0.6, 0.45), (0.7, 0.2, 0.55), (0.5, 0.35, 0.8)]:
+        tot = (s(x, y, z) + s(1 - x, y, z) + s(y, x, z) + s(1 - y, x, z) + s(z, x, y)
+               + s(1 - z, x, y))
+        assert abs(tot - 1.0) < 1e-10
+        assert abs(s(x, y, z) - s(x, 1 - y, z)) < 1e-12  # mirror symmetry in y
