@@ -40,6 +40,17 @@ def spmv_bytes(n: int, nnz: int, value_bytes: int = 8) -> int:
     return (value_bytes + 4) * nnz + 4 * (n + 1) + 2 * value_bytes * n
 
 
+def measured_traffic(N: int):
+    """HBM bytes per launch of the CG SpMV kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary for this workload (calibrated against a
+    kernel of known size; see the summary for the commands), else None."""
+    path = os.path.join(ROOT, "profiles", f"spmv_cg_{N}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("traffic_bytes_per_launch")
+
+
 def cpu_baseline(A_dev, b_dev, n, nnz, iters: int):
     """Serial oracle CG (restatement of ref/ cg_solve + matvec_std) on the same
     300^3 matrix, `iters` iterations; scaled to iterations/s."""
@@ -176,7 +187,7 @@ def main():
                    "nx": N, "ny": N, "nz": N, "nrows": n, "nnz": nnz,
                    "cg_iters_per_step": args.iters},
         "roofline": {"bound": "hbm", "achieved": gbs_cg, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs_cg / HBM_PEAK_GBS, "traffic": None,
+                     "frac": gbs_cg / HBM_PEAK_GBS, "traffic": measured_traffic(N),
                      "kernel": "spmv_stream_kernel<false,true> (CG SpMV + p.Ap partials)",
                      "bytes_per_launch": byt, "avg_ms": ms_cg},
         "spmv_GBs": gbs_cg,

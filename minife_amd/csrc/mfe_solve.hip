@@ -22,8 +22,11 @@ namespace mfe {
 // ---------------------------------------------------------------- SpMV ------
 
 // ZERO_Y: start each row sum at 0*y[row] (matvec_std's beta*ycoefs[row]); the
-// fast CG path starts at +0.0 and does not read y.  DOT: also emit workgroup
-// partials of x.y, y.y and x.x (pAp, Ap.Ap, p.p for CG) at partials[0|1|2][tile].
+// fast CG path starts at +0.0 and does not read y.  DOT: also emit the
+// workgroup partial of x.y (p.Ap in CG) at partials[tile].
+// Every load whose address does not depend on the tile's column stream (row
+// offsets, y[row], x[row]) is issued first, so no memory round trip sits
+// between the LDS barrier and the row sums.
 template <bool ZERO_Y, bool DOT>
 __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(
     int n, const int* __restrict__ ro, const int* __restrict__ cols,
@@ -32,17 +35,22 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(
   if (done && *done) return;
   __shared__ double prod[kSpmvMaxNnz];
   __shared__ double red[4];
-  const int tiles = (int)gridDim.x;
-  const int tile = xcd_contiguous((int)blockIdx.x, tiles);
+  const int tile = (int)blockIdx.x;
   const int r0 = tile * kSpmvRows;
   const int r1 = min(r0 + kSpmvRows, n);
+  const int t = (int)threadIdx.x;
+  const int row = r0 + t;
+  const bool valid = row < r1;
+  const int rr = valid ? row : r1 - 1;
+  const int ra = ro[rr], rb = ro[rr + 1];
+  const double y0 = ZERO_Y ? y[rr] : 0.0;
+  const double xr = DOT ? x[rr] : 0.0;
   const int s = ro[r0];
   const int cnt = ro[r1] - s;
-  const int t = (int)threadIdx.x;
   if (cnt > 0) {
-    // Issue every column and value load of the tile before the first use
-    // (indices clamped into the tile, so all loads are unconditional), then
-    // every x gather, then the products: 27 loads in flight per lane.
+    // Every column and value load of the tile before the first use (indices
+    // clamped into the tile, so all loads are unconditional), then every x
+    // gather, then the products: 27 loads in flight per lane.
     constexpr int K = kSpmvMaxNnz / kBlock;
     int c[K];
     double v[K], xv[K];
@@ -62,11 +70,10 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(
     }
   }
   __syncthreads();
-  const int row = r0 + t;
   double sum = 0.0;
-  if (row < r1) {
-    if (ZERO_Y) sum = 0.0 * y[row];
-    const int a = ro[row] - s, b = ro[row + 1] - s;
+  if (valid) {
+    if (ZERO_Y) sum = 0.0 * y0;
+    const int a = ra - s, b = rb - s;
     int j = a;
     for (; j + 4 <= b; j += 4) {  // LDS reads ahead, additions still in order
       const double p0 = prod[j], p1 = prod[j + 1], p2 = prod[j + 2], p3 = prod[j + 3];
@@ -79,15 +86,8 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(
     y[row] = sum;
   }
   if (DOT) {
-    const double xr = row < r1 ? x[row] : 0.0;
-    const double v0 = block_sum(row < r1 ? sum * xr : 0.0, red);
-    const double v1 = block_sum(row < r1 ? sum * sum : 0.0, red);
-    const double v2 = block_sum(row < r1 ? xr * xr : 0.0, red);
-    if (t == 0) {
-      partials[tile] = v0;
-      partials[tiles + tile] = v1;
-      partials[2 * tiles + tile] = v2;
-    }
+    const double v0 = block_sum(valid ? sum * xr : 0.0, red);
+    if (t == 0) partials[tile] = v0;
   }
 }
 
@@ -108,20 +108,12 @@ __global__ __launch_bounds__(kBlock) void spmv_rowwise_kernel(
     y[row] = sum;
   }
   if (DOT) {
-    const int tiles = (int)gridDim.x;
-    const double xr = row < n ? x[row] : 0.0;
-    const double v0 = block_sum(row < n ? sum * xr : 0.0, red);
-    const double v1 = block_sum(row < n ? sum * sum : 0.0, red);
-    const double v2 = block_sum(row < n ? xr * xr : 0.0, red);
-    if (threadIdx.x == 0) {
-      partials[blockIdx.x] = v0;
-      partials[tiles + blockIdx.x] = v1;
-      partials[2 * tiles + blockIdx.x] = v2;
-    }
+    const double v0 = block_sum(row < n ? sum * x[row] : 0.0, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v0;
   }
 }
 
-// Launch y = A x on stream; returns the number of partial tiles.
+// Launch y = A x on stream; *tiles_out = number of partials written (DOT).
 template <bool ZERO_Y, bool DOT>
 static int launch_spmv(const mfe_csr* A, const double* x, double* y, double* partials,
                        const int* done, hipStream_t stream, int* tiles_out) {
@@ -154,25 +146,57 @@ __global__ __launch_bounds__(kBlock) void waxpby_kernel(int64_t n, double alpha,
 
 // Per-thread partial sums over a fixed grid-stride pattern, then the fixed
 // wave/workgroup tree: deterministic for a given n.
+__device__ __forceinline__ double grid_dot(int64_t n, const double* __restrict__ x,
+                                           const double* __restrict__ y) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    acc += x[i] * y[i];
+  return acc;
+}
+
 __global__ __launch_bounds__(kBlock) void dot_partial_kernel(int64_t n,
                                                              const double* __restrict__ x,
                                                              const double* __restrict__ y,
                                                              double* __restrict__ partials) {
   __shared__ double red[4];
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  double acc = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    acc += x[i] * y[i];
-  acc = block_sum(acc, red);
+  const double acc = block_sum(grid_dot(n, x, y), red);
   if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
 
-// Sum nvec arrays of np partials (array v at partials[v*np]) into out[v].
+// Workgroup sum of p[0..np): each thread adds its strided values in index
+// order (eight loads in flight per step), then the fixed workgroup tree.
 __device__ __forceinline__ double sum_partials(const double* __restrict__ p, int np,
                                                double* red) {
   double acc = 0.0;
-  for (int i = (int)threadIdx.x; i < np; i += kBlock) acc += p[i];
+  int i = (int)threadIdx.x;
+  for (; i + 7 * kBlock < np; i += 8 * kBlock) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = p[i + u * kBlock];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc += v[u];
+  }
+  for (; i < np; i += kBlock) acc += p[i];
   return block_sum(acc, red);
+}
+
+// First stage of a two-stage reduction of nvec arrays of np partials: block j
+// sums partials[v*np + j*kStage .. +kStage) into out[v*gridDim.x + j].
+constexpr int kStage = 8 * kBlock;
+__global__ __launch_bounds__(kBlock) void reduce_stage_kernel(const double* __restrict__ partials,
+                                                              int np, int nvec,
+                                                              double* __restrict__ out,
+                                                              const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ double red[4];
+  const int j = (int)blockIdx.x;
+  const int lo = j * kStage;
+  const int m = min(kStage, np - lo);
+  for (int v = 0; v < nvec; ++v) {
+    const double s = sum_partials(partials + (int64_t)v * np + lo, m, red);
+    if (threadIdx.x == 0) out[(int64_t)v * gridDim.x + j] = s;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void finalize_kernel(const double* __restrict__ partials,
@@ -215,11 +239,24 @@ __global__ __launch_bounds__(kBlock) void dot_seq_kernel(int64_t n, const double
 }
 
 // ------------------------------------------------------------------- CG -----
+//
+// Per iteration k (fast mode), all on one stream:
+//   cg_p       x += alpha_{k-1} p_{k-1} (deferred, see below); p = 1*r + beta*y
+//   spmv       Ap = A p, workgroup partials of p.Ap
+//   reduce     first stage of the p.Ap reduction
+//   alpha      p.Ap; breakdown test; alpha = rtrans / p.Ap
+//   norms(+fin) only when p.Ap < brkdown_tol: ||Ap||, ||p|| for breakdown()
+//   update     r = 1*r + (-alpha)*Ap, workgroup partials of r.r
+//   next       end of k; loop test; rtrans, beta, normr of k+1
+// The reference updates x right after r (cg_solve.hpp:211-212).  Here the
+// update x = 1*x + alpha*p of iteration k is applied by the next cg_p, which
+// reads p anyway (or by cg_finish after the last iteration): the same
+// operation on the same operands, so the same bits, one vector pass fewer.
 
 struct CgState {
   double rtrans, oldrtrans, pAp, alpha, beta, normr, brkdown_tol, tol;
-  double rr;  // r.r of the latest update (fast mode)
-  int k, max_iter, done, status, num_iters, pad;
+  double alpha_prev;  // alpha of the iteration whose x update is pending
+  int k, max_iter, done, status, num_iters, need_norms, x_pending, pad;
 };
 
 // Prologue scalar step: rtrans = r0.r0, normr, and the loop test for k = 1.
@@ -231,10 +268,12 @@ __device__ void cg_begin_first(CgState* st, double rr) {
   if (!(1 <= st->max_iter && st->normr > st->tol)) st->done = 1;
 }
 
-// End of iteration k (update applied) and start of k+1: num_iters = k, the
-// reference's loop test with the normr of iteration k, then rtrans, beta and
-// normr of iteration k+1 (cg_solve.hpp:152-165).
+// End of iteration k (r updated): its x update becomes pending, num_iters = k,
+// then the reference's loop test with the normr of iteration k, and rtrans,
+// beta, normr of iteration k+1 (cg_solve.hpp:152-165).
 __device__ void cg_next(CgState* st, double rr) {
+  st->x_pending = 1;
+  st->alpha_prev = st->alpha;
   st->num_iters = st->k;
   const int k1 = st->k + 1;
   if (!(k1 <= st->max_iter && st->normr > st->tol)) {
@@ -248,25 +287,46 @@ __device__ void cg_next(CgState* st, double rr) {
   st->k = k1;
 }
 
-// Breakdown test and alpha (cg_solve.hpp:186-201, breakdown() :49-67).
-__device__ void cg_alpha(CgState* st, double pAp, double ApAp, double pp, double* hist) {
+__device__ double* hist_row(const CgState* st, double* hist) {
+  return hist + 4 * (int64_t)(st->k - 1);
+}
+
+// Record p.Ap and, unless the breakdown test needs the norms, alpha.
+// Returns 1 when alpha was set.  (cg_solve.hpp:186-201)
+__device__ int cg_alpha_or_defer(CgState* st, double pAp, double* hist) {
+  st->x_pending = 0;  // cg_p of this iteration has applied it
   st->pAp = pAp;
-  double* h = hist + 4 * (int64_t)(st->k - 1);
+  double* h = hist_row(st, hist);
   h[0] = st->rtrans;
   h[1] = pAp;
   h[2] = 0.0;
   h[3] = st->beta;
   if (pAp < st->brkdown_tol) {
-    const double vnorm = sqrt(ApAp), wnorm = sqrt(pp);
-    if (pAp < 0 || fabs(pAp) <= 100 * vnorm * wnorm * DBL_EPSILON) {
+    if (pAp < 0) {
       st->status = MFE_BREAKDOWN;
       st->done = 1;
-      return;
+      return 0;
     }
-    st->brkdown_tol = 0.1 * pAp;
+    st->need_norms = 1;
+    return 0;
   }
   st->alpha = st->rtrans / pAp;
   h[2] = st->alpha;
+  return 1;
+}
+
+// breakdown() (cg_solve.hpp:49-67) with v = Ap, w = p, then alpha.
+__device__ void cg_alpha_after_norms(CgState* st, double ApAp, double pp, double* hist) {
+  st->need_norms = 0;
+  const double vnorm = sqrt(ApAp), wnorm = sqrt(pp);
+  if (fabs(st->pAp) <= 100 * vnorm * wnorm * DBL_EPSILON) {
+    st->status = MFE_BREAKDOWN;
+    st->done = 1;
+    return;
+  }
+  st->brkdown_tol = 0.1 * st->pAp;
+  st->alpha = st->rtrans / st->pAp;
+  hist_row(st, hist)[2] = st->alpha;
 }
 
 __global__ void cg_init_kernel(CgState* st, int max_iter, double tol) {
@@ -293,9 +353,31 @@ __global__ __launch_bounds__(kBlock) void cg_alpha_fast_kernel(CgState* st,
   if (st->done) return;
   __shared__ double red[4];
   const double pAp = sum_partials(partials, np, red);
-  const double ApAp = sum_partials(partials + np, np, red);
-  const double pp = sum_partials(partials + 2 * (int64_t)np, np, red);
-  if (threadIdx.x == 0) cg_alpha(st, pAp, ApAp, pp, hist);
+  if (threadIdx.x == 0) cg_alpha_or_defer(st, pAp, hist);
+}
+
+// ||Ap||^2 and ||p||^2 partials, only when the breakdown test needs them.
+__global__ __launch_bounds__(kBlock) void cg_norms_kernel(int64_t n, const CgState* st,
+                                                          const double* __restrict__ Ap,
+                                                          const double* __restrict__ p,
+                                                          double* __restrict__ partials) {
+  if (st->done || !st->need_norms) return;
+  __shared__ double red[4];
+  const double a = block_sum(grid_dot(n, Ap, Ap), red);
+  const double b = block_sum(grid_dot(n, p, p), red);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = a;
+    partials[gridDim.x + blockIdx.x] = b;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void cg_norms_fin_kernel(CgState* st, const double* partials,
+                                                              int np, double* hist) {
+  if (st->done || !st->need_norms) return;
+  __shared__ double red[4];
+  const double ApAp = sum_partials(partials, np, red);
+  const double pp = sum_partials(partials + np, np, red);
+  if (threadIdx.x == 0) cg_alpha_after_norms(st, ApAp, pp, hist);
 }
 
 __global__ __launch_bounds__(kBlock) void cg_next_fast_kernel(CgState* st,
@@ -314,8 +396,8 @@ __global__ __launch_bounds__(kBlock) void cg_first_seq_kernel(CgState* st, int64
   if (threadIdx.x == 0) cg_begin_first(st, rr);
 }
 
-// End of iteration k: num_iters = k; then the loop test for k+1 and, only if
-// it passes, the sequential r.r of iteration k+1.
+// End of iteration k, the loop test for k+1 and, only if it passes, the
+// sequential r.r of iteration k+1.
 __global__ __launch_bounds__(kBlock) void cg_next_seq_kernel(CgState* st, int64_t n,
                                                              const double* r) {
   __shared__ double buf[kSeqChunk];
@@ -323,6 +405,8 @@ __global__ __launch_bounds__(kBlock) void cg_next_seq_kernel(CgState* st, int64_
   if (threadIdx.x == 0) {
     go = 0;
     if (!st->done) {
+      st->x_pending = 1;
+      st->alpha_prev = st->alpha;
       st->num_iters = st->k;
       const int k1 = st->k + 1;
       if (!(k1 <= st->max_iter && st->normr > st->tol)) st->done = 1;
@@ -347,44 +431,62 @@ __global__ __launch_bounds__(kBlock) void cg_alpha_seq_kernel(CgState* st, int64
   __shared__ double buf[kSeqChunk];
   __shared__ int need;
   if (st->done) return;
-  const double pAp_t0 = dot_sequential(n, Ap, p, buf);
-  if (threadIdx.x == 0) need = pAp_t0 < st->brkdown_tol && !(pAp_t0 < 0);
-  __syncthreads();
-  double ApAp = 0, pp = 0;
-  if (need) {  // breakdown(): sqrt(dot(v,v)), sqrt(dot(w,w)) with v = Ap, w = p
-    ApAp = dot_sequential(n, Ap, Ap, buf);
-    pp = dot_sequential(n, p, p, buf);
+  const double pAp = dot_sequential(n, Ap, p, buf);
+  if (threadIdx.x == 0) {
+    cg_alpha_or_defer(st, pAp, hist);
+    need = st->need_norms;
   }
-  if (threadIdx.x == 0) cg_alpha(st, pAp_t0, ApAp, pp, hist);
+  __syncthreads();
+  if (!need) return;
+  const double ApAp = dot_sequential(n, Ap, Ap, buf);
+  const double pp = dot_sequential(n, p, p, buf);
+  if (threadIdx.x == 0) cg_alpha_after_norms(st, ApAp, pp, hist);
 }
 
-// p = 1*r + beta*y (y = r for k = 1, y = p afterwards; cg_solve.hpp:153-161).
+// x = 1*x + alpha_prev*p if an x update is pending; then p = 1*r + beta*y
+// (y = r for k = 1, y = p afterwards; cg_solve.hpp:153-161, 211).
 __global__ __launch_bounds__(kBlock) void cg_p_kernel(int64_t n, const CgState* st,
                                                       const double* r, const double* y,
-                                                      double* p) {
+                                                      double* p, double* x) {
   if (st->done) return;
   const double beta = st->beta;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    p[i] = 1.0 * r[i] + beta * y[i];
+  const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (st->x_pending) {
+    const double a = st->alpha_prev;
+    for (int64_t i = i0; i < n; i += stride) {
+      const double pi = p[i];
+      x[i] = 1.0 * x[i] + a * pi;
+      p[i] = 1.0 * r[i] + beta * (y == p ? pi : y[i]);
+    }
+  } else {
+    for (int64_t i = i0; i < n; i += stride) p[i] = 1.0 * r[i] + beta * y[i];
+  }
 }
 
-// x = 1*x + alpha*p ; r = 1*r + (-alpha)*Ap (cg_solve.hpp:211-212); with DOT,
-// workgroup partials of the new r.r.
+// After the last iteration: apply the pending x update.
+__global__ __launch_bounds__(kBlock) void cg_finish_kernel(int64_t n, const CgState* st,
+                                                           const double* __restrict__ p,
+                                                           double* __restrict__ x) {
+  if (!st->x_pending) return;
+  const double a = st->alpha_prev;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    x[i] = 1.0 * x[i] + a * p[i];
+}
+
+// r = 1*r + (-alpha)*Ap (cg_solve.hpp:212); with DOT, partials of the new r.r.
 template <bool DOT>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(int64_t n, const CgState* st,
-                                                           double* __restrict__ x,
-                                                           const double* __restrict__ p,
                                                            double* __restrict__ r,
                                                            const double* __restrict__ Ap,
                                                            double* __restrict__ partials) {
   if (st->done) return;
   __shared__ double red[4];
-  const double alpha = st->alpha, malpha = -st->alpha;
+  const double malpha = -st->alpha;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   double acc = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    x[i] = 1.0 * x[i] + alpha * p[i];
     const double ri = 1.0 * r[i] + malpha * Ap[i];
     r[i] = ri;
     if (DOT) acc += ri * ri;
@@ -400,6 +502,8 @@ struct CgWorkspace {
   double *r = nullptr, *p = nullptr, *Ap = nullptr;
   double* partials = nullptr;
   int64_t partials_cap = 0;
+  double* red2 = nullptr;  // second-stage partials (ceil(tiles / kStage))
+  int64_t red2_cap = 0;
   CgState* st = nullptr;
   CgState* h_st = nullptr;  // pinned
   double* hist = nullptr;
@@ -430,6 +534,7 @@ static void ws_free(CgWorkspace* w) {
   hipFree(w->p);
   hipFree(w->Ap);
   hipFree(w->partials);
+  hipFree(w->red2);
   hipFree(w->st);
   hipFree(w->hist);
   if (w->h_st) hipHostFree(w->h_st);
@@ -466,6 +571,15 @@ static int ws_prepare(mfe_ctx* ctx, int64_t n, int64_t tiles, int max_iter, CgWo
     MFE_CHECK(ctx, hipMalloc(&w->partials, sizeof(double) * (size_t)need));
     w->partials_cap = need;
   }
+  const int64_t need2 = cdiv(tiles, kStage) > 0 ? cdiv(tiles, kStage) : 1;
+  if (w->red2_cap < need2) {
+    ws_drop_graph(w);
+    hipFree(w->red2);
+    w->red2 = nullptr;
+    w->red2_cap = 0;
+    MFE_CHECK(ctx, hipMalloc(&w->red2, sizeof(double) * (size_t)need2));
+    w->red2_cap = need2;
+  }
   const int64_t hneed = 4 * (int64_t)(max_iter > 0 ? max_iter : 1);
   if (w->hist_cap < hneed) {
     ws_drop_graph(w);
@@ -484,23 +598,27 @@ static unsigned vec_grid(int64_t n) {
   return (unsigned)(b < kVecBlocks ? (b > 0 ? b : 1) : kVecBlocks);
 }
 
-// One CG iteration body (k >= 2 when y_for_p == p).  Returns a hipError_t.
+// One CG iteration body (k = 1 when y_for_p == r, k >= 2 when y_for_p == p).
 static hipError_t enqueue_iteration(const mfe_csr* A, CgWorkspace* w, double* x,
                                     const double* y_for_p, bool replay, hipStream_t s) {
   const int64_t n = w->n;
   const unsigned vg = vec_grid(n);
-  cg_p_kernel<<<vg, kBlock, 0, s>>>(n, w->st, w->r, y_for_p, w->p);
+  cg_p_kernel<<<vg, kBlock, 0, s>>>(n, w->st, w->r, y_for_p, w->p, x);
   int tiles = 0;
   const int* done = &w->st->done;
   if (replay) {
     launch_spmv<true, false>(A, w->p, w->Ap, nullptr, done, s, &tiles);
     cg_alpha_seq_kernel<<<1, kBlock, 0, s>>>(w->st, n, w->Ap, w->p, w->hist);
-    cg_update_kernel<false><<<vg, kBlock, 0, s>>>(n, w->st, x, w->p, w->r, w->Ap, nullptr);
+    cg_update_kernel<false><<<vg, kBlock, 0, s>>>(n, w->st, w->r, w->Ap, nullptr);
     cg_next_seq_kernel<<<1, kBlock, 0, s>>>(w->st, n, w->r);
   } else {
     launch_spmv<false, true>(A, w->p, w->Ap, w->partials, done, s, &tiles);
-    cg_alpha_fast_kernel<<<1, kBlock, 0, s>>>(w->st, w->partials, tiles, w->hist);
-    cg_update_kernel<true><<<vg, kBlock, 0, s>>>(n, w->st, x, w->p, w->r, w->Ap, w->partials);
+    const int nb2 = (int)cdiv(tiles, kStage);
+    reduce_stage_kernel<<<nb2, kBlock, 0, s>>>(w->partials, tiles, 1, w->red2, done);
+    cg_alpha_fast_kernel<<<1, kBlock, 0, s>>>(w->st, w->red2, nb2, w->hist);
+    cg_norms_kernel<<<vg, kBlock, 0, s>>>(n, w->st, w->Ap, w->p, w->partials);
+    cg_norms_fin_kernel<<<1, kBlock, 0, s>>>(w->st, w->partials, (int)vg, w->hist);
+    cg_update_kernel<true><<<vg, kBlock, 0, s>>>(n, w->st, w->r, w->Ap, w->partials);
     cg_next_fast_kernel<<<1, kBlock, 0, s>>>(w->st, w->partials, (int)vg);
   }
   return hipGetLastError();
@@ -551,7 +669,7 @@ int mfe_matvec_and_dot(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec
   if ((st = check_vec(ctx, x, A->nrows, "matvec x")) || (st = check_vec(ctx, y, A->nrows, "matvec y")))
     return st;
   const int64_t tiles = cdiv(A->nrows, kSpmvRows);
-  if ((st = ensure_partials(ctx, 3 * (tiles > 0 ? tiles : 1)))) return st;
+  if ((st = ensure_partials(ctx, tiles > 0 ? tiles : 1))) return st;
   int nt = 0;
   int e = launch_spmv<true, true>(A, x->d, y->d, ctx->d_partials, nullptr, ctx->stream, &nt);
   if (e) return hip_fail(ctx, (hipError_t)e, "spmv_dot");
@@ -575,7 +693,7 @@ int mfe_time_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y
   if ((st = check_vec(ctx, x, A->nrows, "matvec x")) || (st = check_vec(ctx, y, A->nrows, "matvec y")))
     return st;
   const int64_t tiles = cdiv(A->nrows, kSpmvRows);
-  if ((st = ensure_partials(ctx, 3 * (tiles > 0 ? tiles : 1)))) return st;
+  if ((st = ensure_partials(ctx, tiles > 0 ? tiles : 1))) return st;
   MFE_CHECK(ctx, hipEventRecord(ctx->t_start, ctx->stream));
   for (int i = 0; i < reps; ++i) {
     int e = variant == 0
@@ -723,8 +841,6 @@ int mfe_cg_solve(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x, i
         const int chunks = (int)cdiv(max_iter - 1, kGraphChunk);
         for (int c = 0; c < chunks; ++c) {
           MFE_CHECK(ctx, hipGraphLaunch(w->exec, s));
-          // Check the done flag once a second chunk is queued: the host reads
-          // chunk c-1's flag while chunk c runs.
           MFE_CHECK(ctx, hipMemcpyAsync(w->h_st, w->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
           if (c + 1 < chunks) {
             MFE_CHECK(ctx, hipStreamSynchronize(s));
@@ -733,6 +849,8 @@ int mfe_cg_solve(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x, i
         }
       }
     }
+    cg_finish_kernel<<<vg, kBlock, 0, s>>>(n, w->st, w->p, x->d);
+    MFE_LAUNCHED(ctx, "cg_finish_kernel");
   }
   MFE_CHECK(ctx, hipMemcpyAsync(w->h_st, w->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
   MFE_CHECK(ctx, hipEventRecord(ctx->t_stop, s));
@@ -746,8 +864,8 @@ int mfe_cg_solve(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x, i
     for (int i = 0; i < 5; ++i) times[i] = 0;
     times[MFE_T_TOTAL] = 1e-3 * ms;
   }
-  if (history && hs.num_iters + (hs.status ? 1 : 0) > 0) {
-    const int rows = hs.num_iters + (hs.status == MFE_BREAKDOWN ? 1 : 0);
+  const int rows = hs.num_iters + (hs.status == MFE_BREAKDOWN ? 1 : 0);
+  if (history && rows > 0) {
     MFE_CHECK(ctx, hipMemcpy(history, w->hist, sizeof(double) * 4 * (size_t)rows,
                              hipMemcpyDeviceToHost));
   }

@@ -185,6 +185,20 @@ def test_matvec_dot_waxpby_bit_exact(mfe, ctx, oracle_mod, dims):
     assert bits_equal(w.to_numpy(), oracle_mod.waxpby(0.75, xh, -1.25, want))
 
 
+@pytest.mark.parametrize("n", [1, 1000, 300_001, 3_000_017])
+def test_fast_dot_error_bound(mfe, ctx, n):
+    """The tree dot is within its own error bound of the exact sum (fsum):
+    each thread adds ceil(n / (2048*256)) terms, then 8 + 2 + 8 tree levels."""
+    import math
+    rng = np.random.default_rng(n)
+    xh = rng.standard_normal(n)
+    yh = rng.standard_normal(n)
+    d = mfe.dot(mfe.Vector.from_numpy(xh, ctx), mfe.Vector.from_numpy(yh, ctx))
+    exact = math.fsum((xh * yh).tolist())  # products rounded, as in the kernels
+    depth = -(-n // (2048 * 256)) + 18 + 2
+    assert abs(d - exact) <= depth * EPS * float(np.abs(xh * yh).sum())
+
+
 def test_long_rows_take_the_rowwise_path(mfe, ctx, oracle_mod):
     """A 300-row matrix whose first tile holds more than 27*256 non-zeros."""
     say("case long rows")
@@ -249,18 +263,38 @@ def test_cg_replay_medium(mfe, ctx, dims):
     assert sha(x.to_numpy()) == g["sha256"]["x"]
 
 
+def check_fast_history(ours, ref, n, kmax=7):
+    """Fast-mode scalar history vs the reference's (replay mode is the
+    bit-exact mode and is checked separately).  Iteration 1 differs only by
+    the summation order of its dot products: the reference's left-to-right
+    sum has error <= (n-1) eps sum|x_i y_i|, the tree sum far less, and the
+    terms here are all of one sign, so 2 n eps relative bounds the difference.
+    Iteration 1 then cancels the unit Dirichlet rows (r_1 = 1 - alpha_1 there,
+    ~1e-3), amplifying that difference ~1e3x per such step; later iterations
+    get only a loose sanity bound that still catches any algorithmic error."""
+    worst = []
+    for k in range(kmax):
+        tol = 2 * n * EPS if k == 0 else 1e-6
+        rel = 0.0
+        for col_ours, col_ref in ((1, 0), (0, 1), (2, 2)):
+            d = abs(ours[k, col_ours] - ref[k, col_ref]) / abs(ref[k, col_ref])
+            rel = max(rel, d)
+            assert d <= tol, (k + 1, col_ref, d)
+        worst.append(rel)
+    say("fast-mode max relative deviation per iteration: " +
+        " ".join(f"{w:.1e}" for w in worst))
+
+
 def test_cg_fast_mode_30(mfe, ctx):
-    """Fast mode: tree-reduced dots.  Scalar history within 1e-12 for the
-    iterations where the reference's own trajectory is stable (1-7), and the
-    converged solution within 1e-12 of the reference's."""
+    """Fast mode: tree-reduced dots.  Scalar history close to the reference's
+    over iterations 1-7 (check_fast_history) and the converged solution within
+    1e-12 of the reference's."""
     say("case cg fast 30^3")
     dims = (30, 30, 30)
     g = golden_json(dims)
     h = np.array(g["history"])
     mesh, A, b, x, r = _solve(mfe, ctx, dims, max_iter=200)
-    for k in range(7):
-        for col_ours, col_ref in ((1, 0), (0, 1), (2, 2)):
-            assert abs(r.history[k, col_ours] - h[k, col_ref]) <= 1e-12 * abs(h[k, col_ref])
+    check_fast_history(r.history, h, A.nrows)
     # The reference's solution is pinned by its digest; replay mode
     # reproduces it, and the fast solution must agree with it.
     xr = mfe.Vector(A.nrows, ctx)
@@ -368,7 +402,18 @@ def test_full_size_300(mfe, ctx):
     say("case 300^3 cg fast 12")
     x2 = mfe.Vector(A.nrows, ctx)
     r2 = mfe.cg_solve(A, b, x2, max_iter=g["max_iters"], history=True)
-    h = np.array(g["history"])
-    for k in range(7):
-        for col_ours, col_ref in ((1, 0), (0, 1), (2, 2)):
-            assert abs(r2.history[k, col_ours] - h[k, col_ref]) <= 1e-12 * abs(h[k, col_ref])
+    # At 300^3 the boundary-row cancellation amplifies summation-order
+    # differences quickly (~3e-3 relative by iteration 5): iteration 1 is held
+    # to the rigorous bound, iterations 2-3 to the sanity bound.
+    check_fast_history(r2.history, np.array(g["history"]), A.nrows, kmax=3)
+    say("case 300^3 cg fast to 1e-10")
+    x2.fill(0.0)
+    r3 = mfe.cg_solve(A, b, x2, max_iter=5000, tolerance=1e-10, relative=True)
+    y = mfe.Vector(A.nrows, ctx)
+    mfe.matvec(A, x2, y)
+    bh = b.to_numpy()
+    true_rel = np.linalg.norm(bh - y.to_numpy()) / np.linalg.norm(bh)
+    say(f"fast CG: {r3.num_iters} iterations, recursive {r3.normr / np.linalg.norm(bh):.3e}, "
+        f"true relative residual {true_rel:.3e}")
+    assert r3.normr <= 1e-10 * np.linalg.norm(bh) and true_rel <= 2e-10
+    assert mfe.verify_solution(mesh, x2)[0] == 0
