@@ -145,22 +145,50 @@ def main():
     mfe.time_matvec(A, xs, ys, variant=1, reps=3)
     ms_cg = mfe.time_matvec(A, xs, ys, variant=1, reps=args.spmv_reps)
     ms_std = mfe.time_matvec(A, xs, ys, variant=0, reps=args.spmv_reps)
+    ms_32 = mfe.time_matvec(A, xs, ys, variant=2, reps=args.spmv_reps)
     byt = spmv_bytes(n, nnz)
     gbs_cg = byt / (ms_cg * 1e-3) / 1e9
     gbs_std = byt / (ms_std * 1e-3) / 1e9
+    gbs_32 = spmv_bytes(n, nnz, 4) / (ms_32 * 1e-3) / 1e9
     log(f"spmv (cg kernel) {ms_cg:.4f} ms -> {gbs_cg:.1f} GB/s; matvec_std {ms_std:.4f} ms "
-        f"-> {gbs_std:.1f} GB/s")
+        f"-> {gbs_std:.1f} GB/s; fp32 {ms_32:.4f} ms -> {gbs_32:.1f} GB/s")
 
     # ---- time to ||r||/||b|| <= 1e-10 ----------------------------------------
     tol_run = None
     if not args.no_tol_run:
-        x.fill(0.0)
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        r2 = mfe.cg_solve(A, b, x, max_iter=5000, tolerance=1e-10, relative=True)
-        tt = time.perf_counter() - t0
-        tol_run = {"fp64_s": tt, "fp64_iters": r2.num_iters, "fp64_normr": r2.normr}
-        log(f"time to 1e-10 (fp64): {tt:.3f}s, {r2.num_iters} iterations")
+        bnorm = float(np.sqrt(mfe.dot(b, b)))
+        # FP64 plain CG to ||r||/||b|| <= 1e-10 (warm run first, then timed)
+        for timed in (False, True):
+            x.fill(0.0)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            r2 = mfe.cg_solve(A, b, x, max_iter=5000, tolerance=1e-10, relative=True)
+            tt = time.perf_counter() - t0
+        x64 = x.to_numpy()
+        # mixed: FP32 inner CG to 1e-5 + FP64 refinement to 1e-10
+        xm = mfe.Vector(n, ctx)
+        for timed in (False, True):
+            xm.fill(0.0)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            rm = mfe.cg_solve_mixed(A, b, xm, inner_tol=1e-5, outer_tol=1e-10)
+            tm = time.perf_counter() - t0
+        xmh = xm.to_numpy()
+        # both residuals recomputed in FP64 on the GPU
+        res = mfe.Vector(n, ctx)
+        mfe.matvec(A, x, res)
+        rel64 = float(np.linalg.norm(b.to_numpy() - res.to_numpy())) / bnorm
+        mfe.matvec(A, xm, res)
+        relm = float(np.linalg.norm(b.to_numpy() - res.to_numpy())) / bnorm
+        err = float(np.linalg.norm(xmh - x64) / np.linalg.norm(x64))
+        tol_run = {"fp64_s": tt, "fp64_iters": r2.num_iters, "fp64_true_rel_residual": rel64,
+                   "mixed_s": tm, "mixed_outer": rm.outer_iters, "mixed_inner": rm.inner_iters,
+                   "mixed_true_rel_residual": relm, "mixed_converged": rm.converged,
+                   "mixed_rel_err_vs_fp64": err, "speedup_mixed_vs_fp64": tt / tm}
+        log(f"time to 1e-10: fp64 {tt:.3f}s ({r2.num_iters} its, true rel {rel64:.2e}); mixed "
+            f"{tm:.3f}s ({rm.outer_iters} outer, {rm.inner_iters} inner, true rel {relm:.2e}), "
+            f"|x_mixed-x_fp64|/|x_fp64| = {err:.2e}")
+        del res
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -192,6 +220,7 @@ def main():
                      "bytes_per_launch": byt, "avg_ms": ms_cg},
         "spmv_GBs": gbs_cg,
         "spmv_std_GBs": gbs_std,
+        "spmv_fp32_GBs": gbs_32,
         "assembly": {"struct_s": t_struct, "fe_s": t_asm, "dirichlet_s": t_bc,
                      "elements_per_s": ne / t_asm},
         "time_to_1e-10": tol_run,

@@ -155,11 +155,30 @@ int mfe_cg_solve(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x, i
                  double tolerance, unsigned flags, int* num_iters, double* normr,
                  double* times, double* history);
 
+/* ---- mixed precision (extension; no counterpart in the reference) --------
+ * Iterative refinement around cg_solve: the inner solve is unpreconditioned
+ * CG entirely in FP32 (FP32 copy of A's values sharing the int32 columns;
+ * SpMV products, row sums, axpys and dot products in FP32) on the residual
+ * scaled to unit norm, run to ||r_in|| <= inner_tol (looser, a tenth of the
+ * remaining reduction, when that is all that is still needed) or max_inner
+ * iterations; the outer loop computes r = b - A x, x += ||r|| d and the norms
+ * in FP64 on the GPU, until ||b - A x||_2 <= outer_tol * ||b||_2 or max_outer
+ * refinements.  Returns MFE_OK when converged, MFE_BREAKDOWN if an inner
+ * solve broke down, MFE_UNSUPPORTED if it did not converge in max_outer. */
+int mfe_cg_solve_mixed(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x,
+                       int max_outer, int max_inner, double inner_tol, double outer_tol,
+                       int* outer_iters, int* inner_iters, double* rel_residual, double* times);
+/* y = A x evaluated as the mixed solve's inner SpMV: FP32 values, x rounded
+ * to FP32, FP32 products summed in stored column order, y widened to FP64. */
+int mfe_matvec_fp32(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y);
+
 /* Kernel timing mode, the counterpart of time_kernels (time_kernels.hpp:42-134):
  * launches `reps` back-to-back SpMV kernels y = A x on ctx's stream between two
  * hipEvents and returns the mean kernel time in ms.  variant 0 = matvec_std's
  * kernel (row sums start at 0*y[row]); variant 1 = the kernel inside the fast
- * cg_solve (sums start at +0.0, plus workgroup partials of x.y, y.y, x.x). */
+ * cg_solve (sums start at +0.0, plus workgroup partials of x.y); variant 2 =
+ * the FP32 kernel inside mfe_cg_solve_mixed's inner CG (x is used as its
+ * FP32 copy, converted before the timed launches). */
 int mfe_time_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y, int variant,
                     int reps, double* avg_ms);
 

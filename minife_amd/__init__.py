@@ -45,7 +45,8 @@ EXPORTS = [
     "mfe_generate_matrix_structure", "mfe_perform_element_loop", "mfe_assemble_FE_data",
     "mfe_impose_dirichlet", "mfe_make_local_matrix",
     "mfe_matvec", "mfe_matvec_and_dot", "mfe_dot", "mfe_dot_replay", "mfe_waxpby",
-    "mfe_cg_solve", "mfe_time_matvec", "mfe_soln", "mfe_verify_solution",
+    "mfe_cg_solve", "mfe_cg_solve_mixed", "mfe_matvec_fp32", "mfe_time_matvec", "mfe_soln",
+    "mfe_verify_solution",
 ]
 
 
@@ -104,6 +105,11 @@ def lib() -> ctypes.CDLL:
         "mfe_cg_solve": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_uint,
                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double), _vp, _vp],
                          ctypes.c_int),
+        "mfe_cg_solve_mixed": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                ctypes.c_double, ctypes.POINTER(ctypes.c_int),
+                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                                _vp], ctypes.c_int),
+        "mfe_matvec_fp32": ([_vp, _vp, _vp, _vp], ctypes.c_int),
         "mfe_time_matvec": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
                              ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "mfe_soln": ([ctypes.c_double] * 3 + [ctypes.c_int, ctypes.c_int], ctypes.c_double),
@@ -420,9 +426,44 @@ def cg_solve(A: CSRMatrix, b: Vector, x: Vector, max_iter: int = 200,
     return CGResult(ni.value, nr.value, st, times, hist)
 
 
+@dataclass
+class MixedResult:
+    outer_iters: int
+    inner_iters: int
+    rel_residual: float
+    status: int
+    seconds: float
+
+    @property
+    def converged(self) -> bool:
+        return self.status == MFE_OK
+
+
+def cg_solve_mixed(A: CSRMatrix, b: Vector, x: Vector, *, max_outer: int = 20,
+                   max_inner: int = 2000, inner_tol: float = 1e-5,
+                   outer_tol: float = 1e-10) -> MixedResult:
+    """Mixed-precision iterative refinement around cg_solve (FP32 inner CG on an
+    FP32 copy of A, FP64 residual/update/norms on the GPU); see minife_hip.h."""
+    oi, ii = ctypes.c_int(0), ctypes.c_int(0)
+    rel = ctypes.c_double(0)
+    times = np.zeros(5, np.float64)
+    st = lib().mfe_cg_solve_mixed(A.ctx.h, A.h, b.h, x.h, int(max_outer), int(max_inner),
+                                  float(inner_tol), float(outer_tol), ctypes.byref(oi),
+                                  ctypes.byref(ii), ctypes.byref(rel), _ptr(times))
+    if st not in (MFE_OK, MFE_BREAKDOWN, MFE_UNSUPPORTED):
+        A.ctx.check(st, "cg_solve_mixed")
+    return MixedResult(oi.value, ii.value, rel.value, st, float(times[T_TOTAL]))
+
+
+def matvec_fp32(A: CSRMatrix, x: Vector, y: Vector) -> None:
+    """y = A x as the mixed solve's FP32 inner SpMV (result widened to FP64)."""
+    A.ctx.check(lib().mfe_matvec_fp32(A.ctx.h, A.h, x.h, y.h), "matvec_fp32")
+
+
 def time_matvec(A: CSRMatrix, x: Vector, y: Vector, variant: int = 1, reps: int = 20) -> float:
     """Mean SpMV kernel time in ms over `reps` back-to-back launches (HIP events
-    on the context's stream); variant 1 is the kernel inside the fast cg_solve."""
+    on the context's stream); variant 1 is the kernel inside the fast cg_solve,
+    variant 2 the FP32 kernel inside cg_solve_mixed."""
     ms = ctypes.c_double(0)
     A.ctx.check(lib().mfe_time_matvec(A.ctx.h, A.h, x.h, y.h, int(variant), int(reps),
                                       ctypes.byref(ms)), "time_matvec")

@@ -139,8 +139,11 @@ static void csr_release(mfe_csr* A) {
   hipFree(A->row_offsets);
   hipFree(A->cols);
   hipFree(A->vals);
+  hipFree(A->vals32);
   A->rows = A->row_offsets = A->cols = nullptr;
   A->vals = nullptr;
+  A->vals32 = nullptr;
+  A->vals32_valid = false;
   A->nrows = A->nnz = 0;
   A->max_block_nnz = -1;
   A->has_local_indices = false;

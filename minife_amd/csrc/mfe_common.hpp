@@ -45,6 +45,8 @@ struct mfe_csr {
   int* row_offsets = nullptr;
   int* cols = nullptr;
   double* vals = nullptr;
+  float* vals32 = nullptr;  // FP32 copy for the mixed-precision solve (lazy)
+  bool vals32_valid = false;
   bool has_local_indices = false;
   int64_t num_cols = 0;
   // SpMV plan: largest nnz of any kSpmvRows-row block (-1 = not computed).
@@ -93,7 +95,8 @@ __device__ __forceinline__ int xcd_contiguous(int bid, int n) {
 }
 
 // Wave64 sum in a fixed butterfly order (deterministic).
-__device__ __forceinline__ double wave_sum(double v) {
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
   return v;
@@ -101,12 +104,13 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Workgroup sum for kBlock threads: wave butterflies, then the four wave
 // totals added in wave order by thread 0.  Result valid in thread 0.
-__device__ __forceinline__ double block_sum(double v, double* lds4) {
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* lds4) {
   v = wave_sum(v);
   const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   if (lane == 0) lds4[wave] = v;
   __syncthreads();
-  double t = 0;
+  T t = 0;
   if (threadIdx.x == 0) t = ((lds4[0] + lds4[1]) + lds4[2]) + lds4[3];
   __syncthreads();
   return t;

@@ -1,4 +1,4 @@
-// CSR SpMV, vector kernels and the conjugate-gradient solver on gfx950.
+// CSR SpMV, vector kernels and the conjugate-gradient solvers on gfx950.
 //
 // SpMV ("row-stream"): a workgroup owns 256 consecutive rows.  Its threads
 // stream the rows' value/column arrays with coalesced loads, form every
@@ -10,10 +10,14 @@
 // CG: one HIP stream, device-resident scalars, fused kernels, and chunks of
 // iterations captured in a HIP graph and replayed; every kernel first checks a
 // device "done" flag so the reference's data-dependent loop test
-// (cg_solve.hpp:152) stops the work without host round trips.
+// (cg_solve.hpp:152) stops the work without host round trips.  The kernels are
+// templated on the vector type: double for cg_solve, float for the inner solve
+// of the mixed-precision refinement.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 #include "mfe_common.hpp"
 
@@ -21,20 +25,23 @@ namespace mfe {
 
 // ---------------------------------------------------------------- SpMV ------
 
-// ZERO_Y: start each row sum at 0*y[row] (matvec_std's beta*ycoefs[row]); the
-// fast CG path starts at +0.0 and does not read y.  DOT: also emit the
-// workgroup partial of x.y (p.Ap in CG) at partials[tile].
-// Every load whose address does not depend on the tile's column stream (row
-// offsets, y[row], x[row]) is issued first, so no memory round trip sits
+enum SpmvMode { kPlain = 0, kDot = 1, kResid = 2 };
+
+// MODE kPlain: y = A x.  kDot: also the workgroup partial of x.y (p.Ap in CG)
+// at partials[tile].  kResid: y = 1*b + (-1)*(A x) (the reference's r = b - Ap
+// waxpby, fused) and the partial of y.y.  ZERO_Y: start each row sum at
+// 0*y[row] (matvec_std's beta*ycoefs[row]); otherwise at +0.0 without reading
+// y.  Loads whose addresses do not depend on the column stream (row offsets,
+// y[row], x[row], b[row]) are issued first, so no memory round trip sits
 // between the LDS barrier and the row sums.
-template <bool ZERO_Y, bool DOT>
+template <typename T, bool ZERO_Y, int MODE>
 __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(
     int n, const int* __restrict__ ro, const int* __restrict__ cols,
-    const double* __restrict__ vals, const double* __restrict__ x, double* __restrict__ y,
-    double* __restrict__ partials, const int* __restrict__ done) {
+    const T* __restrict__ vals, const T* __restrict__ x, T* __restrict__ y,
+    const T* __restrict__ bvec, T* __restrict__ partials, const int* __restrict__ done) {
   if (done && *done) return;
-  __shared__ double prod[kSpmvMaxNnz];
-  __shared__ double red[4];
+  __shared__ T prod[kSpmvMaxNnz];
+  __shared__ T red[4];
   const int tile = (int)blockIdx.x;
   const int r0 = tile * kSpmvRows;
   const int r1 = min(r0 + kSpmvRows, n);
@@ -43,8 +50,9 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(
   const bool valid = row < r1;
   const int rr = valid ? row : r1 - 1;
   const int ra = ro[rr], rb = ro[rr + 1];
-  const double y0 = ZERO_Y ? y[rr] : 0.0;
-  const double xr = DOT ? x[rr] : 0.0;
+  const T y0 = ZERO_Y ? y[rr] : T(0);
+  const T xr = MODE == kDot ? x[rr] : T(0);
+  const T br = MODE == kResid ? bvec[rr] : T(0);
   const int s = ro[r0];
   const int cnt = ro[r1] - s;
   if (cnt > 0) {
@@ -53,7 +61,7 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(
     // gather, then the products: 27 loads in flight per lane.
     constexpr int K = kSpmvMaxNnz / kBlock;
     int c[K];
-    double v[K], xv[K];
+    T v[K], xv[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int i = t + k * kBlock;
@@ -70,53 +78,58 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(
     }
   }
   __syncthreads();
-  double sum = 0.0;
+  T sum = T(0);
   if (valid) {
-    if (ZERO_Y) sum = 0.0 * y0;
+    if (ZERO_Y) sum = T(0) * y0;
     const int a = ra - s, b = rb - s;
     int j = a;
     for (; j + 4 <= b; j += 4) {  // LDS reads ahead, additions still in order
-      const double p0 = prod[j], p1 = prod[j + 1], p2 = prod[j + 2], p3 = prod[j + 3];
+      const T p0 = prod[j], p1 = prod[j + 1], p2 = prod[j + 2], p3 = prod[j + 3];
       sum += p0;
       sum += p1;
       sum += p2;
       sum += p3;
     }
     for (; j < b; ++j) sum += prod[j];
+    if (MODE == kResid) sum = T(1) * br + T(-1) * sum;
     y[row] = sum;
   }
-  if (DOT) {
-    const double v0 = block_sum(valid ? sum * xr : 0.0, red);
+  if (MODE != kPlain) {
+    const T term = MODE == kDot ? sum * xr : sum * sum;
+    const T v0 = block_sum<T>(valid ? term : T(0), red);
     if (t == 0) partials[tile] = v0;
   }
 }
 
 // Fallback for matrices with a 256-row tile of more than 27*256 non-zeros:
 // one thread per row, same per-row order, so the same bits.
-template <bool ZERO_Y, bool DOT>
+template <typename T, bool ZERO_Y, int MODE>
 __global__ __launch_bounds__(kBlock) void spmv_rowwise_kernel(
     int n, const int* __restrict__ ro, const int* __restrict__ cols,
-    const double* __restrict__ vals, const double* __restrict__ x, double* __restrict__ y,
-    double* __restrict__ partials, const int* __restrict__ done) {
+    const T* __restrict__ vals, const T* __restrict__ x, T* __restrict__ y,
+    const T* __restrict__ bvec, T* __restrict__ partials, const int* __restrict__ done) {
   if (done && *done) return;
-  __shared__ double red[4];
+  __shared__ T red[4];
   const int row = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  double sum = 0.0;
+  T sum = T(0);
   if (row < n) {
-    if (ZERO_Y) sum = 0.0 * y[row];
+    if (ZERO_Y) sum = T(0) * y[row];
     for (int j = ro[row]; j < ro[row + 1]; ++j) sum += vals[j] * x[cols[j]];
+    if (MODE == kResid) sum = T(1) * bvec[row] + T(-1) * sum;
     y[row] = sum;
   }
-  if (DOT) {
-    const double v0 = block_sum(row < n ? sum * x[row] : 0.0, red);
+  if (MODE != kPlain) {
+    const T term = row < n ? (MODE == kDot ? sum * x[row] : sum * sum) : T(0);
+    const T v0 = block_sum<T>(term, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = v0;
   }
 }
 
-// Launch y = A x on stream; *tiles_out = number of partials written (DOT).
-template <bool ZERO_Y, bool DOT>
-static int launch_spmv(const mfe_csr* A, const double* x, double* y, double* partials,
-                       const int* done, hipStream_t stream, int* tiles_out) {
+// y = A x (values `vals`, FP64 or the FP32 copy) on stream; *tiles_out = the
+// number of partials written.
+template <typename T, bool ZERO_Y, int MODE>
+static int launch_spmv(const mfe_csr* A, const T* vals, const T* x, T* y, const T* bvec,
+                       T* partials, const int* done, hipStream_t stream, int* tiles_out) {
   const int n = (int)A->nrows;
   if (n == 0) {
     if (tiles_out) *tiles_out = 0;
@@ -124,11 +137,11 @@ static int launch_spmv(const mfe_csr* A, const double* x, double* y, double* par
   }
   const int tiles = (int)cdiv(n, kSpmvRows);
   if (A->max_block_nnz >= 0 && A->max_block_nnz <= kSpmvMaxNnz) {
-    spmv_stream_kernel<ZERO_Y, DOT><<<tiles, kBlock, 0, stream>>>(
-        n, A->row_offsets, A->cols, A->vals, x, y, partials, done);
+    spmv_stream_kernel<T, ZERO_Y, MODE><<<tiles, kBlock, 0, stream>>>(
+        n, A->row_offsets, A->cols, vals, x, y, bvec, partials, done);
   } else {
-    spmv_rowwise_kernel<ZERO_Y, DOT><<<tiles, kBlock, 0, stream>>>(
-        n, A->row_offsets, A->cols, A->vals, x, y, partials, done);
+    spmv_rowwise_kernel<T, ZERO_Y, MODE><<<tiles, kBlock, 0, stream>>>(
+        n, A->row_offsets, A->cols, vals, x, y, bvec, partials, done);
   }
   if (tiles_out) *tiles_out = tiles;
   return (int)hipGetLastError();
@@ -136,65 +149,91 @@ static int launch_spmv(const mfe_csr* A, const double* x, double* y, double* par
 
 // -------------------------------------------------------------- vectors -----
 
-__global__ __launch_bounds__(kBlock) void waxpby_kernel(int64_t n, double alpha,
-                                                        const double* x, double beta,
-                                                        const double* y, double* w) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void waxpby_kernel(int64_t n, T alpha, const T* x, T beta,
+                                                        const T* y, T* w) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     w[i] = alpha * x[i] + beta * y[i];
 }
 
+// dst = (float)(src * scale)
+__global__ __launch_bounds__(kBlock) void to_f32_kernel(int64_t n, const double* __restrict__ src,
+                                                        double scale, float* __restrict__ dst) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = (float)(src[i] * scale);
+}
+
+// dst = (double)src
+__global__ __launch_bounds__(kBlock) void to_f64_kernel(int64_t n, const float* __restrict__ src,
+                                                        double* __restrict__ dst) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = (double)src[i];
+}
+
+// x = 1*x + a*(double)d : the refinement update in FP64.
+__global__ __launch_bounds__(kBlock) void refine_update_kernel(int64_t n, double a,
+                                                               const float* __restrict__ d,
+                                                               double* __restrict__ x) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    x[i] = 1.0 * x[i] + a * (double)d[i];
+}
+
 // Per-thread partial sums over a fixed grid-stride pattern, then the fixed
 // wave/workgroup tree: deterministic for a given n.
-__device__ __forceinline__ double grid_dot(int64_t n, const double* __restrict__ x,
-                                           const double* __restrict__ y) {
+template <typename T>
+__device__ __forceinline__ T grid_dot(int64_t n, const T* __restrict__ x, const T* __restrict__ y) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  double acc = 0.0;
+  T acc = T(0);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     acc += x[i] * y[i];
   return acc;
 }
 
-__global__ __launch_bounds__(kBlock) void dot_partial_kernel(int64_t n,
-                                                             const double* __restrict__ x,
-                                                             const double* __restrict__ y,
-                                                             double* __restrict__ partials) {
-  __shared__ double red[4];
-  const double acc = block_sum(grid_dot(n, x, y), red);
+template <typename T>
+__global__ __launch_bounds__(kBlock) void dot_partial_kernel(int64_t n, const T* __restrict__ x,
+                                                             const T* __restrict__ y,
+                                                             T* __restrict__ partials) {
+  __shared__ T red[4];
+  const T acc = block_sum<T>(grid_dot<T>(n, x, y), red);
   if (threadIdx.x == 0) partials[blockIdx.x] = acc;
 }
 
 // Workgroup sum of p[0..np): each thread adds its strided values in index
 // order (eight loads in flight per step), then the fixed workgroup tree.
-__device__ __forceinline__ double sum_partials(const double* __restrict__ p, int np,
-                                               double* red) {
-  double acc = 0.0;
+template <typename T>
+__device__ __forceinline__ T sum_partials(const T* __restrict__ p, int np, T* red) {
+  T acc = T(0);
   int i = (int)threadIdx.x;
   for (; i + 7 * kBlock < np; i += 8 * kBlock) {
-    double v[8];
+    T v[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) v[u] = p[i + u * kBlock];
 #pragma unroll
     for (int u = 0; u < 8; ++u) acc += v[u];
   }
   for (; i < np; i += kBlock) acc += p[i];
-  return block_sum(acc, red);
+  return block_sum<T>(acc, red);
 }
 
 // First stage of a two-stage reduction of nvec arrays of np partials: block j
 // sums partials[v*np + j*kStage .. +kStage) into out[v*gridDim.x + j].
 constexpr int kStage = 8 * kBlock;
-__global__ __launch_bounds__(kBlock) void reduce_stage_kernel(const double* __restrict__ partials,
+template <typename T>
+__global__ __launch_bounds__(kBlock) void reduce_stage_kernel(const T* __restrict__ partials,
                                                               int np, int nvec,
-                                                              double* __restrict__ out,
+                                                              T* __restrict__ out,
                                                               const int* __restrict__ done) {
   if (done && *done) return;
-  __shared__ double red[4];
+  __shared__ T red[4];
   const int j = (int)blockIdx.x;
   const int lo = j * kStage;
   const int m = min(kStage, np - lo);
   for (int v = 0; v < nvec; ++v) {
-    const double s = sum_partials(partials + (int64_t)v * np + lo, m, red);
+    const T s = sum_partials<T>(partials + (int64_t)v * np + lo, m, red);
     if (threadIdx.x == 0) out[(int64_t)v * gridDim.x + j] = s;
   }
 }
@@ -204,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void finalize_kernel(const double* __restri
                                                           double* __restrict__ out) {
   __shared__ double red[4];
   for (int v = 0; v < nvec; ++v) {
-    const double s = sum_partials(partials + (int64_t)v * np, np, red);
+    const double s = sum_partials<double>(partials + (int64_t)v * np, np, red);
     if (threadIdx.x == 0) out[v] = s;
   }
 }
@@ -252,6 +291,7 @@ __global__ __launch_bounds__(kBlock) void dot_seq_kernel(int64_t n, const double
 // update x = 1*x + alpha*p of iteration k is applied by the next cg_p, which
 // reads p anyway (or by cg_finish after the last iteration): the same
 // operation on the same operands, so the same bits, one vector pass fewer.
+// Scalars (rtrans, alpha, beta, ...) are kept in double for both vector types.
 
 struct CgState {
   double rtrans, oldrtrans, pAp, alpha, beta, normr, brkdown_tol, tol;
@@ -292,8 +332,8 @@ __device__ double* hist_row(const CgState* st, double* hist) {
 }
 
 // Record p.Ap and, unless the breakdown test needs the norms, alpha.
-// Returns 1 when alpha was set.  (cg_solve.hpp:186-201)
-__device__ int cg_alpha_or_defer(CgState* st, double pAp, double* hist) {
+// (cg_solve.hpp:186-201)
+__device__ void cg_alpha_or_defer(CgState* st, double pAp, double* hist) {
   st->x_pending = 0;  // cg_p of this iteration has applied it
   st->pAp = pAp;
   double* h = hist_row(st, hist);
@@ -305,14 +345,13 @@ __device__ int cg_alpha_or_defer(CgState* st, double pAp, double* hist) {
     if (pAp < 0) {
       st->status = MFE_BREAKDOWN;
       st->done = 1;
-      return 0;
+      return;
     }
     st->need_norms = 1;
-    return 0;
+    return;
   }
   st->alpha = st->rtrans / pAp;
   h[2] = st->alpha;
-  return 1;
 }
 
 // breakdown() (cg_solve.hpp:49-67) with v = Ap, w = p, then alpha.
@@ -340,55 +379,59 @@ __global__ void cg_init_kernel(CgState* st, int max_iter, double tol) {
 }
 
 // Fast mode scalar kernels (one workgroup each) ------------------------------
-__global__ __launch_bounds__(kBlock) void cg_first_fast_kernel(CgState* st,
-                                                               const double* partials, int np) {
-  __shared__ double red[4];
-  const double rr = sum_partials(partials, np, red);
-  if (threadIdx.x == 0) cg_begin_first(st, rr);
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_first_fast_kernel(CgState* st, const T* partials,
+                                                               int np) {
+  __shared__ T red[4];
+  const T rr = sum_partials<T>(partials, np, red);
+  if (threadIdx.x == 0) cg_begin_first(st, (double)rr);
 }
 
-__global__ __launch_bounds__(kBlock) void cg_alpha_fast_kernel(CgState* st,
-                                                               const double* partials, int np,
-                                                               double* hist) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_alpha_fast_kernel(CgState* st, const T* partials,
+                                                               int np, double* hist) {
   if (st->done) return;
-  __shared__ double red[4];
-  const double pAp = sum_partials(partials, np, red);
-  if (threadIdx.x == 0) cg_alpha_or_defer(st, pAp, hist);
+  __shared__ T red[4];
+  const T pAp = sum_partials<T>(partials, np, red);
+  if (threadIdx.x == 0) cg_alpha_or_defer(st, (double)pAp, hist);
 }
 
 // ||Ap||^2 and ||p||^2 partials, only when the breakdown test needs them.
+template <typename T>
 __global__ __launch_bounds__(kBlock) void cg_norms_kernel(int64_t n, const CgState* st,
-                                                          const double* __restrict__ Ap,
-                                                          const double* __restrict__ p,
-                                                          double* __restrict__ partials) {
+                                                          const T* __restrict__ Ap,
+                                                          const T* __restrict__ p,
+                                                          T* __restrict__ partials) {
   if (st->done || !st->need_norms) return;
-  __shared__ double red[4];
-  const double a = block_sum(grid_dot(n, Ap, Ap), red);
-  const double b = block_sum(grid_dot(n, p, p), red);
+  __shared__ T red[4];
+  const T a = block_sum<T>(grid_dot<T>(n, Ap, Ap), red);
+  const T b = block_sum<T>(grid_dot<T>(n, p, p), red);
   if (threadIdx.x == 0) {
     partials[blockIdx.x] = a;
     partials[gridDim.x + blockIdx.x] = b;
   }
 }
 
-__global__ __launch_bounds__(kBlock) void cg_norms_fin_kernel(CgState* st, const double* partials,
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_norms_fin_kernel(CgState* st, const T* partials,
                                                               int np, double* hist) {
   if (st->done || !st->need_norms) return;
-  __shared__ double red[4];
-  const double ApAp = sum_partials(partials, np, red);
-  const double pp = sum_partials(partials + np, np, red);
-  if (threadIdx.x == 0) cg_alpha_after_norms(st, ApAp, pp, hist);
+  __shared__ T red[4];
+  const T ApAp = sum_partials<T>(partials, np, red);
+  const T pp = sum_partials<T>(partials + np, np, red);
+  if (threadIdx.x == 0) cg_alpha_after_norms(st, (double)ApAp, (double)pp, hist);
 }
 
-__global__ __launch_bounds__(kBlock) void cg_next_fast_kernel(CgState* st,
-                                                              const double* partials, int np) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_next_fast_kernel(CgState* st, const T* partials,
+                                                              int np) {
   if (st->done) return;
-  __shared__ double red[4];
-  const double rr = sum_partials(partials, np, red);
-  if (threadIdx.x == 0) cg_next(st, rr);
+  __shared__ T red[4];
+  const T rr = sum_partials<T>(partials, np, red);
+  if (threadIdx.x == 0) cg_next(st, (double)rr);
 }
 
-// Replay mode scalar kernels (sequential dots) ---------------------------------
+// Replay mode scalar kernels (sequential dots, FP64 only) -------------------------
 __global__ __launch_bounds__(kBlock) void cg_first_seq_kernel(CgState* st, int64_t n,
                                                               const double* r) {
   __shared__ double buf[kSeqChunk];
@@ -445,103 +488,138 @@ __global__ __launch_bounds__(kBlock) void cg_alpha_seq_kernel(CgState* st, int64
 
 // x = 1*x + alpha_prev*p if an x update is pending; then p = 1*r + beta*y
 // (y = r for k = 1, y = p afterwards; cg_solve.hpp:153-161, 211).
-__global__ __launch_bounds__(kBlock) void cg_p_kernel(int64_t n, const CgState* st,
-                                                      const double* r, const double* y,
-                                                      double* p, double* x) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_p_kernel(int64_t n, const CgState* st, const T* r,
+                                                      const T* y, T* p, T* x) {
   if (st->done) return;
-  const double beta = st->beta;
+  const T beta = (T)st->beta;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (st->x_pending) {
-    const double a = st->alpha_prev;
+    const T a = (T)st->alpha_prev;
     for (int64_t i = i0; i < n; i += stride) {
-      const double pi = p[i];
-      x[i] = 1.0 * x[i] + a * pi;
-      p[i] = 1.0 * r[i] + beta * (y == p ? pi : y[i]);
+      const T pi = p[i];
+      x[i] = T(1) * x[i] + a * pi;
+      p[i] = T(1) * r[i] + beta * (y == p ? pi : y[i]);
     }
   } else {
-    for (int64_t i = i0; i < n; i += stride) p[i] = 1.0 * r[i] + beta * y[i];
+    for (int64_t i = i0; i < n; i += stride) p[i] = T(1) * r[i] + beta * y[i];
   }
 }
 
 // After the last iteration: apply the pending x update.
+template <typename T>
 __global__ __launch_bounds__(kBlock) void cg_finish_kernel(int64_t n, const CgState* st,
-                                                           const double* __restrict__ p,
-                                                           double* __restrict__ x) {
+                                                           const T* __restrict__ p,
+                                                           T* __restrict__ x) {
   if (!st->x_pending) return;
-  const double a = st->alpha_prev;
+  const T a = (T)st->alpha_prev;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    x[i] = 1.0 * x[i] + a * p[i];
+    x[i] = T(1) * x[i] + a * p[i];
 }
 
 // r = 1*r + (-alpha)*Ap (cg_solve.hpp:212); with DOT, partials of the new r.r.
-template <bool DOT>
+template <typename T, bool DOT>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(int64_t n, const CgState* st,
-                                                           double* __restrict__ r,
-                                                           const double* __restrict__ Ap,
-                                                           double* __restrict__ partials) {
+                                                           T* __restrict__ r,
+                                                           const T* __restrict__ Ap,
+                                                           T* __restrict__ partials) {
   if (st->done) return;
-  __shared__ double red[4];
-  const double malpha = -st->alpha;
+  __shared__ T red[4];
+  const T malpha = (T)(-st->alpha);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  double acc = 0.0;
+  T acc = T(0);
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const double ri = 1.0 * r[i] + malpha * Ap[i];
+    const T ri = T(1) * r[i] + malpha * Ap[i];
     r[i] = ri;
     if (DOT) acc += ri * ri;
   }
   if (DOT) {
-    acc = block_sum(acc, red);
+    acc = block_sum<T>(acc, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = acc;
   }
 }
 
-struct CgWorkspace {
+// --------------------------------------------------------- workspace -----
+
+template <typename T>
+struct VecSet {
   int64_t n = -1;
-  double *r = nullptr, *p = nullptr, *Ap = nullptr;
-  double* partials = nullptr;
-  int64_t partials_cap = 0;
-  double* red2 = nullptr;  // second-stage partials (ceil(tiles / kStage))
-  int64_t red2_cap = 0;
+  T *r = nullptr, *p = nullptr, *Ap = nullptr;
+  T* partials = nullptr;  // 3 x max(tiles, kVecBlocks)
+  T* red2 = nullptr;      // ceil(tiles / kStage)
+  void release() {
+    hipFree(r);
+    hipFree(p);
+    hipFree(Ap);
+    hipFree(partials);
+    hipFree(red2);
+    r = p = Ap = partials = red2 = nullptr;
+    n = -1;
+  }
+};
+
+struct GraphSlot {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const void *kA = nullptr, *kvals = nullptr, *kb = nullptr, *kx = nullptr;
+  unsigned kflags = 0;
+  void drop() {
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    kA = kvals = kb = kx = nullptr;
+  }
+};
+
+struct CgWorkspace {
+  VecSet<double> v64;
+  VecSet<float> v32;
+  float *f_rhs = nullptr, *f_x = nullptr;  // inner system of the mixed solve
+  int64_t f_n = -1;
   CgState* st = nullptr;
   CgState* h_st = nullptr;  // pinned
   double* hist = nullptr;
   int64_t hist_cap = 0;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  // What the captured graph was built for.
-  const void *kA = nullptr, *kb = nullptr, *kx = nullptr;
-  const double* khist = nullptr;
-  unsigned kflags = 0;
-  int kchunk = 0;
-  int64_t knnz = -1;
+  GraphSlot g64, g32;
 };
 
 constexpr int kGraphChunk = 25;  // CG iterations per captured graph
 
-static void ws_drop_graph(CgWorkspace* w) {
-  if (w->exec) hipGraphExecDestroy(w->exec);
-  if (w->graph) hipGraphDestroy(w->graph);
-  w->exec = nullptr;
-  w->graph = nullptr;
-  w->kA = w->kb = w->kx = nullptr;
-}
-
 static void ws_free(CgWorkspace* w) {
-  ws_drop_graph(w);
-  hipFree(w->r);
-  hipFree(w->p);
-  hipFree(w->Ap);
-  hipFree(w->partials);
-  hipFree(w->red2);
+  w->g64.drop();
+  w->g32.drop();
+  w->v64.release();
+  w->v32.release();
+  hipFree(w->f_rhs);
+  hipFree(w->f_x);
   hipFree(w->st);
   hipFree(w->hist);
   if (w->h_st) hipHostFree(w->h_st);
   delete w;
 }
 
-static int ws_prepare(mfe_ctx* ctx, int64_t n, int64_t tiles, int max_iter, CgWorkspace** out) {
+template <typename T>
+static int vecset_prepare(mfe_ctx* ctx, VecSet<T>& v, int64_t n, GraphSlot& g) {
+  if (v.n == n) return MFE_OK;
+  g.drop();
+  v.release();
+  const int64_t tiles = cdiv(n, kSpmvRows);
+  const size_t bytes = sizeof(T) * (size_t)(n > 0 ? n : 1);
+  MFE_CHECK(ctx, hipMalloc(&v.r, bytes));
+  MFE_CHECK(ctx, hipMalloc(&v.p, bytes));
+  MFE_CHECK(ctx, hipMalloc(&v.Ap, bytes));
+  const int64_t np = 3 * (tiles > kVecBlocks ? tiles : kVecBlocks);
+  MFE_CHECK(ctx, hipMalloc(&v.partials, sizeof(T) * (size_t)np));
+  const int64_t n2 = cdiv(tiles, kStage) > 0 ? cdiv(tiles, kStage) : 1;
+  MFE_CHECK(ctx, hipMalloc(&v.red2, sizeof(T) * (size_t)n2));
+  v.n = n;
+  return MFE_OK;
+}
+
+static int ws_get(mfe_ctx* ctx, int max_iter, CgWorkspace** out) {
   CgWorkspace* w = ctx->cg;
   if (!w) {
     w = new CgWorkspace();
@@ -549,40 +627,10 @@ static int ws_prepare(mfe_ctx* ctx, int64_t n, int64_t tiles, int max_iter, CgWo
     MFE_CHECK(ctx, hipMalloc(&w->st, sizeof(CgState)));
     MFE_CHECK(ctx, hipHostMalloc(&w->h_st, sizeof(CgState), hipHostMallocDefault));
   }
-  if (w->n != n) {
-    ws_drop_graph(w);
-    hipFree(w->r);
-    hipFree(w->p);
-    hipFree(w->Ap);
-    w->r = w->p = w->Ap = nullptr;
-    w->n = -1;
-    const size_t bytes = sizeof(double) * (size_t)(n > 0 ? n : 1);
-    MFE_CHECK(ctx, hipMalloc(&w->r, bytes));
-    MFE_CHECK(ctx, hipMalloc(&w->p, bytes));
-    MFE_CHECK(ctx, hipMalloc(&w->Ap, bytes));
-    w->n = n;
-  }
-  const int64_t need = 3 * (tiles > kVecBlocks ? tiles : kVecBlocks);
-  if (w->partials_cap < need) {
-    ws_drop_graph(w);
-    hipFree(w->partials);
-    w->partials = nullptr;
-    w->partials_cap = 0;
-    MFE_CHECK(ctx, hipMalloc(&w->partials, sizeof(double) * (size_t)need));
-    w->partials_cap = need;
-  }
-  const int64_t need2 = cdiv(tiles, kStage) > 0 ? cdiv(tiles, kStage) : 1;
-  if (w->red2_cap < need2) {
-    ws_drop_graph(w);
-    hipFree(w->red2);
-    w->red2 = nullptr;
-    w->red2_cap = 0;
-    MFE_CHECK(ctx, hipMalloc(&w->red2, sizeof(double) * (size_t)need2));
-    w->red2_cap = need2;
-  }
   const int64_t hneed = 4 * (int64_t)(max_iter > 0 ? max_iter : 1);
   if (w->hist_cap < hneed) {
-    ws_drop_graph(w);
+    w->g64.drop();
+    w->g32.drop();
     hipFree(w->hist);
     w->hist = nullptr;
     w->hist_cap = 0;
@@ -599,29 +647,140 @@ static unsigned vec_grid(int64_t n) {
 }
 
 // One CG iteration body (k = 1 when y_for_p == r, k >= 2 when y_for_p == p).
-static hipError_t enqueue_iteration(const mfe_csr* A, CgWorkspace* w, double* x,
-                                    const double* y_for_p, bool replay, hipStream_t s) {
-  const int64_t n = w->n;
+template <typename T>
+static hipError_t enqueue_iteration(const mfe_csr* A, const T* vals, VecSet<T>& v, CgState* st,
+                                    double* hist, T* x, const T* y_for_p, bool replay,
+                                    hipStream_t s) {
+  const int64_t n = v.n;
   const unsigned vg = vec_grid(n);
-  cg_p_kernel<<<vg, kBlock, 0, s>>>(n, w->st, w->r, y_for_p, w->p, x);
+  cg_p_kernel<T><<<vg, kBlock, 0, s>>>(n, st, v.r, y_for_p, v.p, x);
   int tiles = 0;
-  const int* done = &w->st->done;
-  if (replay) {
-    launch_spmv<true, false>(A, w->p, w->Ap, nullptr, done, s, &tiles);
-    cg_alpha_seq_kernel<<<1, kBlock, 0, s>>>(w->st, n, w->Ap, w->p, w->hist);
-    cg_update_kernel<false><<<vg, kBlock, 0, s>>>(n, w->st, w->r, w->Ap, nullptr);
-    cg_next_seq_kernel<<<1, kBlock, 0, s>>>(w->st, n, w->r);
-  } else {
-    launch_spmv<false, true>(A, w->p, w->Ap, w->partials, done, s, &tiles);
-    const int nb2 = (int)cdiv(tiles, kStage);
-    reduce_stage_kernel<<<nb2, kBlock, 0, s>>>(w->partials, tiles, 1, w->red2, done);
-    cg_alpha_fast_kernel<<<1, kBlock, 0, s>>>(w->st, w->red2, nb2, w->hist);
-    cg_norms_kernel<<<vg, kBlock, 0, s>>>(n, w->st, w->Ap, w->p, w->partials);
-    cg_norms_fin_kernel<<<1, kBlock, 0, s>>>(w->st, w->partials, (int)vg, w->hist);
-    cg_update_kernel<true><<<vg, kBlock, 0, s>>>(n, w->st, w->r, w->Ap, w->partials);
-    cg_next_fast_kernel<<<1, kBlock, 0, s>>>(w->st, w->partials, (int)vg);
+  const int* done = &st->done;
+  if constexpr (std::is_same<T, double>::value) {
+    if (replay) {
+      launch_spmv<double, true, kPlain>(A, vals, v.p, v.Ap, nullptr, nullptr, done, s, &tiles);
+      cg_alpha_seq_kernel<<<1, kBlock, 0, s>>>(st, n, v.Ap, v.p, hist);
+      cg_update_kernel<double, false><<<vg, kBlock, 0, s>>>(n, st, v.r, v.Ap, nullptr);
+      cg_next_seq_kernel<<<1, kBlock, 0, s>>>(st, n, v.r);
+      return hipGetLastError();
+    }
   }
+  launch_spmv<T, false, kDot>(A, vals, v.p, v.Ap, nullptr, v.partials, done, s, &tiles);
+  const int nb2 = (int)cdiv(tiles, kStage);
+  reduce_stage_kernel<T><<<nb2, kBlock, 0, s>>>(v.partials, tiles, 1, v.red2, done);
+  cg_alpha_fast_kernel<T><<<1, kBlock, 0, s>>>(st, v.red2, nb2, hist);
+  cg_norms_kernel<T><<<vg, kBlock, 0, s>>>(n, st, v.Ap, v.p, v.partials);
+  cg_norms_fin_kernel<T><<<1, kBlock, 0, s>>>(st, v.partials, (int)vg, hist);
+  cg_update_kernel<T, true><<<vg, kBlock, 0, s>>>(n, st, v.r, v.Ap, v.partials);
+  cg_next_fast_kernel<T><<<1, kBlock, 0, s>>>(st, v.partials, (int)vg);
   return hipGetLastError();
+}
+
+// Unpreconditioned CG (cg_solve.hpp:69-229) on vectors of type T from the
+// initial x.  Leaves the final state in w->h_st.
+template <typename T>
+static int run_cg(mfe_ctx* ctx, const mfe_csr* A, const T* vals, const T* b, T* x, int max_iter,
+                  double tol, bool replay, bool eager, CgWorkspace* w, VecSet<T>& v,
+                  GraphSlot& slot, const void* key_b, const void* key_x, unsigned key_flags) {
+  hipStream_t s = ctx->stream;
+  const int64_t n = v.n;
+  cg_init_kernel<<<1, 1, 0, s>>>(w->st, max_iter, tol);
+  MFE_LAUNCHED(ctx, "cg_init_kernel");
+  if (n > 0) {
+    const unsigned vg = vec_grid(n);
+    // p = 1*x + 0*x ; Ap = A p ; r = 1*b + (-1)*Ap ; rtrans = r.r  (cg_solve.hpp:124-136)
+    waxpby_kernel<T><<<vg, kBlock, 0, s>>>(n, T(1), x, T(0), x, v.p);
+    // The reference's Ap starts as a zero vector, so its 0*Ap[row] is +0.0:
+    // start the sums at +0.0 instead of reading the workspace.
+    int e = launch_spmv<T, false, kPlain>(A, vals, v.p, v.Ap, nullptr, nullptr, nullptr, s, nullptr);
+    if (e) return hip_fail(ctx, (hipError_t)e, "cg prologue spmv");
+    waxpby_kernel<T><<<vg, kBlock, 0, s>>>(n, T(1), b, T(-1), v.Ap, v.r);
+    bool seq = false;
+    if constexpr (std::is_same<T, double>::value) seq = replay;
+    if (seq) {
+      if constexpr (std::is_same<T, double>::value)
+        cg_first_seq_kernel<<<1, kBlock, 0, s>>>(w->st, n, v.r);
+    } else {
+      dot_partial_kernel<T><<<vg, kBlock, 0, s>>>(n, v.r, v.r, v.partials);
+      cg_first_fast_kernel<T><<<1, kBlock, 0, s>>>(w->st, v.partials, (int)vg);
+    }
+    MFE_LAUNCHED(ctx, "cg prologue");
+    // Iteration 1: p = 1*r + 0*r.
+    hipError_t he = enqueue_iteration<T>(A, vals, v, w->st, w->hist, x, v.r, replay, s);
+    if (he != hipSuccess) return hip_fail(ctx, he, "cg iteration 1");
+    if (max_iter > 1) {
+      if (eager) {
+        for (int k = 2; k <= max_iter; ++k) {
+          he = enqueue_iteration<T>(A, vals, v, w->st, w->hist, x, v.p, replay, s);
+          if (he != hipSuccess) return hip_fail(ctx, he, "cg iteration");
+          if (k % kGraphChunk == 0) {
+            MFE_CHECK(ctx, hipMemcpyAsync(w->h_st, w->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+            MFE_CHECK(ctx, hipStreamSynchronize(s));
+            if (w->h_st->done) break;
+          }
+        }
+      } else {
+        const bool reuse = slot.exec && slot.kA == A && slot.kvals == vals && slot.kb == key_b &&
+                           slot.kx == key_x && slot.kflags == key_flags;
+        if (!reuse) {
+          slot.drop();
+          MFE_CHECK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+          hipError_t ce = hipSuccess;
+          for (int c = 0; c < kGraphChunk && ce == hipSuccess; ++c)
+            ce = enqueue_iteration<T>(A, vals, v, w->st, w->hist, x, v.p, replay, s);
+          hipGraph_t g = nullptr;
+          hipError_t ee = hipStreamEndCapture(s, &g);
+          if (ce != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return hip_fail(ctx, ce, "cg capture");
+          }
+          if (ee != hipSuccess) return hip_fail(ctx, ee, "hipStreamEndCapture");
+          slot.graph = g;
+          MFE_CHECK(ctx, hipGraphInstantiate(&slot.exec, g, nullptr, nullptr, 0));
+          slot.kA = A;
+          slot.kvals = vals;
+          slot.kb = key_b;
+          slot.kx = key_x;
+          slot.kflags = key_flags;
+        }
+        const int chunks = (int)cdiv(max_iter - 1, kGraphChunk);
+        for (int c = 0; c < chunks; ++c) {
+          MFE_CHECK(ctx, hipGraphLaunch(slot.exec, s));
+          MFE_CHECK(ctx, hipMemcpyAsync(w->h_st, w->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+          if (c + 1 < chunks) {
+            MFE_CHECK(ctx, hipStreamSynchronize(s));
+            if (w->h_st->done) break;
+          }
+        }
+      }
+    }
+    cg_finish_kernel<T><<<vg, kBlock, 0, s>>>(n, w->st, v.p, x);
+    MFE_LAUNCHED(ctx, "cg_finish_kernel");
+  }
+  MFE_CHECK(ctx, hipMemcpyAsync(w->h_st, w->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+  MFE_CHECK(ctx, hipStreamSynchronize(s));
+  return MFE_OK;
+}
+
+__global__ void vals_to_f32_kernel(int64_t nnz, const double* __restrict__ v,
+                                   float* __restrict__ v32) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += stride)
+    v32[i] = (float)v[i];
+}
+
+// FP32 copy of A's values (round to nearest), made once per set of values.
+static int ensure_fp32(mfe_ctx* ctx, const mfe_csr* Ac) {
+  mfe_csr* A = const_cast<mfe_csr*>(Ac);
+  if (A->vals32_valid) return MFE_OK;
+  if (!A->vals32) MFE_CHECK(ctx, hipMalloc(&A->vals32, sizeof(float) * (size_t)(A->nnz > 0 ? A->nnz : 1)));
+  if (A->nnz > 0) {
+    vals_to_f32_kernel<<<(unsigned)std::min<int64_t>(cdiv(A->nnz, kBlock), 65536), kBlock, 0,
+                         ctx->stream>>>(A->nnz, A->vals, A->vals32);
+    MFE_LAUNCHED(ctx, "vals_to_f32_kernel");
+  }
+  A->vals32_valid = true;
+  return MFE_OK;
 }
 
 }  // namespace mfe
@@ -652,12 +811,21 @@ static int ensure_partials(mfe_ctx* ctx, int64_t count) {
   return MFE_OK;
 }
 
+static int read_scalar(mfe_ctx* ctx, const double* d, double* out) {
+  MFE_CHECK(ctx, hipMemcpyAsync(ctx->h_scalars, d, sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
+  MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  *out = ctx->h_scalars[0];
+  return MFE_OK;
+}
+
 int mfe_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y) {
   if (!ctx || !A) return MFE_BAD_ARG;
   int st;
   if ((st = check_vec(ctx, x, A->nrows, "matvec x")) || (st = check_vec(ctx, y, A->nrows, "matvec y")))
     return st;
-  int e = launch_spmv<true, false>(A, x->d, y->d, nullptr, nullptr, ctx->stream, nullptr);
+  int e = launch_spmv<double, true, kPlain>(A, A->vals, x->d, y->d, nullptr, nullptr, nullptr,
+                                            ctx->stream, nullptr);
   if (e) return hip_fail(ctx, (hipError_t)e, "spmv");
   return MFE_OK;
 }
@@ -671,7 +839,8 @@ int mfe_matvec_and_dot(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec
   const int64_t tiles = cdiv(A->nrows, kSpmvRows);
   if ((st = ensure_partials(ctx, tiles > 0 ? tiles : 1))) return st;
   int nt = 0;
-  int e = launch_spmv<true, true>(A, x->d, y->d, ctx->d_partials, nullptr, ctx->stream, &nt);
+  int e = launch_spmv<double, true, kDot>(A, A->vals, x->d, y->d, nullptr, ctx->d_partials,
+                                          nullptr, ctx->stream, &nt);
   if (e) return hip_fail(ctx, (hipError_t)e, "spmv_dot");
   if (nt == 0) {
     *result = 0;
@@ -679,27 +848,56 @@ int mfe_matvec_and_dot(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec
   }
   finalize_kernel<<<1, kBlock, 0, ctx->stream>>>(ctx->d_partials, nt, 1, ctx->d_scalars);
   MFE_LAUNCHED(ctx, "finalize_kernel");
-  MFE_CHECK(ctx, hipMemcpyAsync(ctx->h_scalars, ctx->d_scalars, sizeof(double),
-                                hipMemcpyDeviceToHost, ctx->stream));
-  MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-  *result = ctx->h_scalars[0];
+  return read_scalar(ctx, ctx->d_scalars, result);
+}
+
+int mfe_matvec_fp32(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y) {
+  if (!ctx || !A) return MFE_BAD_ARG;
+  int st;
+  if ((st = check_vec(ctx, x, A->nrows, "matvec x")) || (st = check_vec(ctx, y, A->nrows, "matvec y")))
+    return st;
+  const int64_t n = A->nrows;
+  if (n == 0) return MFE_OK;
+  CgWorkspace* w = nullptr;
+  if ((st = ws_get(ctx, 1, &w)) || (st = vecset_prepare<float>(ctx, w->v32, n, w->g32))) return st;
+  if ((st = ensure_fp32(ctx, A))) return st;
+  const unsigned vg = vec_grid(n);
+  to_f32_kernel<<<vg, kBlock, 0, ctx->stream>>>(n, x->d, 1.0, w->v32.p);
+  int e = launch_spmv<float, false, kPlain>(A, A->vals32, w->v32.p, w->v32.Ap, nullptr, nullptr,
+                                            nullptr, ctx->stream, nullptr);
+  if (e) return hip_fail(ctx, (hipError_t)e, "spmv fp32");
+  to_f64_kernel<<<vg, kBlock, 0, ctx->stream>>>(n, w->v32.Ap, y->d);
+  MFE_LAUNCHED(ctx, "to_f64_kernel");
   return MFE_OK;
 }
 
 int mfe_time_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y, int variant,
                     int reps, double* avg_ms) {
-  if (!ctx || !A || !avg_ms || reps <= 0 || variant < 0 || variant > 1) return MFE_BAD_ARG;
+  if (!ctx || !A || !avg_ms || reps <= 0 || variant < 0 || variant > 2) return MFE_BAD_ARG;
   int st;
   if ((st = check_vec(ctx, x, A->nrows, "matvec x")) || (st = check_vec(ctx, y, A->nrows, "matvec y")))
     return st;
-  const int64_t tiles = cdiv(A->nrows, kSpmvRows);
+  const int64_t n = A->nrows;
+  const int64_t tiles = cdiv(n, kSpmvRows);
   if ((st = ensure_partials(ctx, tiles > 0 ? tiles : 1))) return st;
+  CgWorkspace* w = nullptr;
+  if (variant == 2) {
+    if ((st = ws_get(ctx, 1, &w)) || (st = vecset_prepare<float>(ctx, w->v32, n, w->g32))) return st;
+    if ((st = ensure_fp32(ctx, A))) return st;
+    to_f32_kernel<<<vec_grid(n), kBlock, 0, ctx->stream>>>(n, x->d, 1.0, w->v32.p);
+  }
   MFE_CHECK(ctx, hipEventRecord(ctx->t_start, ctx->stream));
   for (int i = 0; i < reps; ++i) {
-    int e = variant == 0
-                ? launch_spmv<true, false>(A, x->d, y->d, nullptr, nullptr, ctx->stream, nullptr)
-                : launch_spmv<false, true>(A, x->d, y->d, ctx->d_partials, nullptr, ctx->stream,
-                                           nullptr);
+    int e;
+    if (variant == 0)
+      e = launch_spmv<double, true, kPlain>(A, A->vals, x->d, y->d, nullptr, nullptr, nullptr,
+                                            ctx->stream, nullptr);
+    else if (variant == 1)
+      e = launch_spmv<double, false, kDot>(A, A->vals, x->d, y->d, nullptr, ctx->d_partials,
+                                           nullptr, ctx->stream, nullptr);
+    else
+      e = launch_spmv<float, false, kDot>(A, A->vals32, w->v32.p, w->v32.Ap, nullptr,
+                                          w->v32.partials, nullptr, ctx->stream, nullptr);
     if (e) return hip_fail(ctx, (hipError_t)e, "spmv");
   }
   MFE_CHECK(ctx, hipEventRecord(ctx->t_stop, ctx->stream));
@@ -717,15 +915,11 @@ int mfe_dot(mfe_ctx* ctx, const mfe_vec* x, const mfe_vec* y, double* result) {
   const unsigned g = vec_grid(n);
   int st;
   if ((st = ensure_partials(ctx, g))) return st;
-  dot_partial_kernel<<<g, kBlock, 0, ctx->stream>>>(n, x->d, y->d, ctx->d_partials);
+  dot_partial_kernel<double><<<g, kBlock, 0, ctx->stream>>>(n, x->d, y->d, ctx->d_partials);
   MFE_LAUNCHED(ctx, "dot_partial_kernel");
   finalize_kernel<<<1, kBlock, 0, ctx->stream>>>(ctx->d_partials, (int)g, 1, ctx->d_scalars);
   MFE_LAUNCHED(ctx, "finalize_kernel");
-  MFE_CHECK(ctx, hipMemcpyAsync(ctx->h_scalars, ctx->d_scalars, sizeof(double),
-                                hipMemcpyDeviceToHost, ctx->stream));
-  MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-  *result = ctx->h_scalars[0];
-  return MFE_OK;
+  return read_scalar(ctx, ctx->d_scalars, result);
 }
 
 int mfe_dot_replay(mfe_ctx* ctx, const mfe_vec* x, const mfe_vec* y, double* result) {
@@ -733,11 +927,7 @@ int mfe_dot_replay(mfe_ctx* ctx, const mfe_vec* x, const mfe_vec* y, double* res
   if (y->n < x->n) return fail(ctx, MFE_BAD_ARG, "dot: y shorter than x");
   dot_seq_kernel<<<1, kBlock, 0, ctx->stream>>>(x->n, x->d, y->d, ctx->d_scalars);
   MFE_LAUNCHED(ctx, "dot_seq_kernel");
-  MFE_CHECK(ctx, hipMemcpyAsync(ctx->h_scalars, ctx->d_scalars, sizeof(double),
-                                hipMemcpyDeviceToHost, ctx->stream));
-  MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-  *result = ctx->h_scalars[0];
-  return MFE_OK;
+  return read_scalar(ctx, ctx->d_scalars, result);
 }
 
 int mfe_waxpby(mfe_ctx* ctx, double alpha, const mfe_vec* x, double beta, const mfe_vec* y,
@@ -745,7 +935,8 @@ int mfe_waxpby(mfe_ctx* ctx, double alpha, const mfe_vec* x, double beta, const 
   if (!ctx || !x || !y || !w) return MFE_BAD_ARG;
   if (y->n < x->n || w->n < x->n) return fail(ctx, MFE_BAD_ARG, "waxpby: y and w must be at least as long as x");
   if (x->n == 0) return MFE_OK;
-  waxpby_kernel<<<vec_grid(x->n), kBlock, 0, ctx->stream>>>(x->n, alpha, x->d, beta, y->d, w->d);
+  waxpby_kernel<double><<<vec_grid(x->n), kBlock, 0, ctx->stream>>>(x->n, alpha, x->d, beta, y->d,
+                                                                    w->d);
   MFE_LAUNCHED(ctx, "waxpby_kernel");
   return MFE_OK;
 }
@@ -763,7 +954,6 @@ int mfe_cg_solve(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x, i
   MFE_CHECK(ctx, hipSetDevice(ctx->device));
   const bool replay = (flags & MFE_CG_REPLAY) != 0;
   const bool eager = (flags & MFE_CG_EAGER) != 0;
-  hipStream_t s = ctx->stream;
 
   double tol = tolerance;
   if (flags & MFE_CG_TOL_RELATIVE) {
@@ -771,90 +961,16 @@ int mfe_cg_solve(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x, i
     if ((st = mfe_dot(ctx, b, b, &bb))) return st;
     tol = tolerance * std::sqrt(bb);
   }
-
-  const int64_t tiles = cdiv(n, kSpmvRows);
   CgWorkspace* w = nullptr;
-  if ((st = ws_prepare(ctx, n, tiles, max_iter, &w))) return st;
-
-  MFE_CHECK(ctx, hipEventRecord(ctx->t_start, s));
-  cg_init_kernel<<<1, 1, 0, s>>>(w->st, max_iter, tol);
-  MFE_LAUNCHED(ctx, "cg_init_kernel");
+  if ((st = ws_get(ctx, max_iter, &w)) || (st = vecset_prepare<double>(ctx, w->v64, n, w->g64)))
+    return st;
   *num_iters = 0;
-  if (n > 0) {
-    const unsigned vg = vec_grid(n);
-    // p = 1*x + 0*x ; Ap = A p ; r = 1*b + (-1)*Ap ; rtrans = r.r  (cg_solve.hpp:124-136)
-    waxpby_kernel<<<vg, kBlock, 0, s>>>(n, 1.0, x->d, 0.0, x->d, w->p);
-    // The reference's Ap starts as a zero vector, so its 0*Ap[row] is +0.0:
-    // start the sums at +0.0 instead of reading the workspace.
-    int e = launch_spmv<false, false>(A, w->p, w->Ap, nullptr, nullptr, s, nullptr);
-    if (e) return hip_fail(ctx, (hipError_t)e, "cg prologue spmv");
-    waxpby_kernel<<<vg, kBlock, 0, s>>>(n, 1.0, b->d, -1.0, w->Ap, w->r);
-    if (replay) {
-      cg_first_seq_kernel<<<1, kBlock, 0, s>>>(w->st, n, w->r);
-    } else {
-      dot_partial_kernel<<<vg, kBlock, 0, s>>>(n, w->r, w->r, w->partials);
-      cg_first_fast_kernel<<<1, kBlock, 0, s>>>(w->st, w->partials, (int)vg);
-    }
-    MFE_LAUNCHED(ctx, "cg prologue");
-    // Iteration 1: p = 1*r + 0*r.
-    hipError_t he = enqueue_iteration(A, w, x->d, w->r, replay, s);
-    if (he != hipSuccess) return hip_fail(ctx, he, "cg iteration 1");
-
-    if (max_iter > 1) {
-      if (eager) {
-        for (int k = 2; k <= max_iter; ++k) {
-          he = enqueue_iteration(A, w, x->d, w->p, replay, s);
-          if (he != hipSuccess) return hip_fail(ctx, he, "cg iteration");
-          if (k % kGraphChunk == 0) {
-            MFE_CHECK(ctx, hipMemcpyAsync(w->h_st, w->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
-            MFE_CHECK(ctx, hipStreamSynchronize(s));
-            if (w->h_st->done) break;
-          }
-        }
-      } else {
-        const bool reuse = w->exec && w->kA == A && w->kb == b && w->kx == x &&
-                           w->khist == w->hist && w->kflags == (flags & MFE_CG_REPLAY) &&
-                           w->kchunk == kGraphChunk && w->knnz == A->nnz;
-        if (!reuse) {
-          ws_drop_graph(w);
-          MFE_CHECK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-          hipError_t ce = hipSuccess;
-          for (int c = 0; c < kGraphChunk && ce == hipSuccess; ++c)
-            ce = enqueue_iteration(A, w, x->d, w->p, replay, s);
-          hipGraph_t g = nullptr;
-          hipError_t ee = hipStreamEndCapture(s, &g);
-          if (ce != hipSuccess) {
-            if (g) hipGraphDestroy(g);
-            return hip_fail(ctx, ce, "cg capture");
-          }
-          if (ee != hipSuccess) return hip_fail(ctx, ee, "hipStreamEndCapture");
-          w->graph = g;
-          MFE_CHECK(ctx, hipGraphInstantiate(&w->exec, g, nullptr, nullptr, 0));
-          w->kA = A;
-          w->kb = b;
-          w->kx = x;
-          w->khist = w->hist;
-          w->kflags = flags & MFE_CG_REPLAY;
-          w->kchunk = kGraphChunk;
-          w->knnz = A->nnz;
-        }
-        const int chunks = (int)cdiv(max_iter - 1, kGraphChunk);
-        for (int c = 0; c < chunks; ++c) {
-          MFE_CHECK(ctx, hipGraphLaunch(w->exec, s));
-          MFE_CHECK(ctx, hipMemcpyAsync(w->h_st, w->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
-          if (c + 1 < chunks) {
-            MFE_CHECK(ctx, hipStreamSynchronize(s));
-            if (w->h_st->done) break;
-          }
-        }
-      }
-    }
-    cg_finish_kernel<<<vg, kBlock, 0, s>>>(n, w->st, w->p, x->d);
-    MFE_LAUNCHED(ctx, "cg_finish_kernel");
-  }
-  MFE_CHECK(ctx, hipMemcpyAsync(w->h_st, w->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
-  MFE_CHECK(ctx, hipEventRecord(ctx->t_stop, s));
-  MFE_CHECK(ctx, hipStreamSynchronize(s));
+  MFE_CHECK(ctx, hipEventRecord(ctx->t_start, ctx->stream));
+  st = run_cg<double>(ctx, A, A->vals, b->d, x->d, max_iter, tol, replay, eager, w, w->v64, w->g64,
+                      b, x, flags & MFE_CG_REPLAY);
+  if (st) return st;
+  MFE_CHECK(ctx, hipEventRecord(ctx->t_stop, ctx->stream));
+  MFE_CHECK(ctx, hipEventSynchronize(ctx->t_stop));
   const CgState hs = *w->h_st;
   *num_iters = hs.num_iters;
   *normr = hs.normr;
@@ -870,6 +986,111 @@ int mfe_cg_solve(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x, i
                              hipMemcpyDeviceToHost));
   }
   return hs.status == MFE_BREAKDOWN ? MFE_BREAKDOWN : MFE_OK;
+}
+
+int mfe_cg_solve_mixed(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x,
+                       int max_outer, int max_inner, double inner_tol, double outer_tol,
+                       int* outer_iters, int* inner_iters, double* rel_residual, double* times) {
+  if (!ctx || !A || !outer_iters || !inner_iters || !rel_residual || max_outer < 0 || max_inner < 0)
+    return MFE_BAD_ARG;
+  if (!A->has_local_indices)
+    return fail(ctx, MFE_NOT_LOCAL,
+                "cg_solve_mixed: A.has_local_indices is false; call make_local_matrix(A) first");
+  const int64_t n = A->nrows;
+  int st;
+  if ((st = check_vec(ctx, b, n, "cg b")) || (st = check_vec(ctx, x, n, "cg x"))) return st;
+  MFE_CHECK(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  *outer_iters = 0;
+  *inner_iters = 0;
+  *rel_residual = 0;
+  CgWorkspace* w = nullptr;
+  if ((st = ws_get(ctx, max_inner, &w)) || (st = vecset_prepare<double>(ctx, w->v64, n, w->g64)) ||
+      (st = vecset_prepare<float>(ctx, w->v32, n, w->g32)))
+    return st;
+  if (w->f_n != n) {
+    hipFree(w->f_rhs);
+    hipFree(w->f_x);
+    w->f_rhs = w->f_x = nullptr;
+    w->g32.drop();
+    MFE_CHECK(ctx, hipMalloc(&w->f_rhs, sizeof(float) * (size_t)(n > 0 ? n : 1)));
+    MFE_CHECK(ctx, hipMalloc(&w->f_x, sizeof(float) * (size_t)(n > 0 ? n : 1)));
+    w->f_n = n;
+  }
+  MFE_CHECK(ctx, hipEventRecord(ctx->t_start, s));
+  if ((st = ensure_fp32(ctx, A))) return st;
+  if (n == 0) return MFE_OK;
+  const unsigned vg = vec_grid(n);
+  const int tiles = (int)cdiv(n, kSpmvRows);
+  double bb = 0;
+  if ((st = mfe_dot(ctx, b, b, &bb))) return st;
+  const double nb = std::sqrt(bb);
+  VecSet<double>& v = w->v64;
+  // r = 1*b + (-1)*(A x) in FP64 with the partials of r.r, fused in the SpMV.
+  auto residual = [&](double* rnorm) -> int {
+    int e = launch_spmv<double, false, kResid>(A, A->vals, x->d, v.r, b->d, v.partials, nullptr,
+                                               s, nullptr);
+    if (e) return hip_fail(ctx, (hipError_t)e, "residual spmv");
+    finalize_kernel<<<1, kBlock, 0, s>>>(v.partials, tiles, 1, ctx->d_scalars);
+    MFE_LAUNCHED(ctx, "finalize_kernel");
+    double rr = 0;
+    int r2 = read_scalar(ctx, ctx->d_scalars, &rr);
+    *rnorm = std::sqrt(rr);
+    return r2;
+  };
+  double rn = 0;
+  if ((st = residual(&rn))) return st;
+  double rel = nb > 0 ? rn / nb : rn;
+  int status = MFE_OK;
+  int outer = 0, inner_total = 0;
+  while (rel > outer_tol) {
+    if (outer >= max_outer) {
+      status = MFE_UNSUPPORTED;
+      fail(ctx, MFE_UNSUPPORTED, "cg_solve_mixed: not converged within max_outer refinements");
+      break;
+    }
+    // Inner system A d = r / ||r|| in FP32 from d = 0, to inner_tol -- or,
+    // when less than that is still needed to reach outer_tol, to a tenth of
+    // the remaining reduction (the inner solve's FP32 recursive residual runs
+    // ahead of the true one by a small factor).
+    const double need = 0.1 * outer_tol / rel;
+    const double tol_k = need > inner_tol ? std::min(need, 0.1) : inner_tol;
+    to_f32_kernel<<<vg, kBlock, 0, s>>>(n, v.r, 1.0 / rn, w->f_rhs);
+    MFE_CHECK(ctx, hipMemsetAsync(w->f_x, 0, sizeof(float) * (size_t)n, s));
+    MFE_LAUNCHED(ctx, "to_f32_kernel");
+    if ((st = run_cg<float>(ctx, A, A->vals32, w->f_rhs, w->f_x, max_inner, tol_k, false,
+                            false, w, w->v32, w->g32, w->f_rhs, w->f_x, 0)))
+      return st;
+    const CgState hs = *w->h_st;
+    inner_total += hs.num_iters;
+    if (hs.num_iters == 0) {
+      status = MFE_BREAKDOWN;
+      fail(ctx, MFE_BREAKDOWN, "cg_solve_mixed: inner solve made no progress");
+      break;
+    }
+    refine_update_kernel<<<vg, kBlock, 0, s>>>(n, rn, w->f_x, x->d);
+    MFE_LAUNCHED(ctx, "refine_update_kernel");
+    ++outer;
+    if ((st = residual(&rn))) return st;
+    rel = nb > 0 ? rn / nb : rn;
+    if (hs.status == MFE_BREAKDOWN && rel > outer_tol) {
+      status = MFE_BREAKDOWN;
+      fail(ctx, MFE_BREAKDOWN, "cg_solve_mixed: inner solve broke down");
+      break;
+    }
+  }
+  MFE_CHECK(ctx, hipEventRecord(ctx->t_stop, s));
+  MFE_CHECK(ctx, hipEventSynchronize(ctx->t_stop));
+  *outer_iters = outer;
+  *inner_iters = inner_total;
+  *rel_residual = rel;
+  if (times) {
+    float ms = 0;
+    MFE_CHECK(ctx, hipEventElapsedTime(&ms, ctx->t_start, ctx->t_stop));
+    for (int i = 0; i < 5; ++i) times[i] = 0;
+    times[MFE_T_TOTAL] = 1e-3 * ms;
+  }
+  return status;
 }
 
 }  // extern "C"

@@ -59,6 +59,7 @@ def lib():
     L.orc_impose_dirichlet.argtypes = [ctypes.c_double, _i64, _i32p, _i32p, _f64p, _f64p,
                                        _i32p, _i64]
     L.orc_matvec.argtypes = [_i64, _i32p, _i32p, _f64p, _f64p, _f64p]
+    L.orc_matvec_f32.argtypes = [_i64, _i32p, _i32p, _f64p, _f64p, _f64p]
     L.orc_dot.argtypes = [_i64, _f64p, _f64p]
     L.orc_dot.restype = ctypes.c_double
     L.orc_waxpby.argtypes = [_i64, ctypes.c_double, _f64p, ctypes.c_double, _f64p, _f64p]
@@ -171,6 +172,12 @@ def matvec(A: CSR, x: np.ndarray, y: np.ndarray | None = None) -> np.ndarray:
     if y is None:
         y = np.zeros(A.nrows, np.float64)
     lib().orc_matvec(A.nrows, A.row_offsets, A.cols, A.vals, np.ascontiguousarray(x), y)
+    return y
+
+
+def matvec_f32(A: CSR, x: np.ndarray) -> np.ndarray:
+    y = np.zeros(A.nrows, np.float64)
+    lib().orc_matvec_f32(A.nrows, A.row_offsets, A.cols, A.vals, np.ascontiguousarray(x), y)
     return y
 
 

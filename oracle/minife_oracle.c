@@ -401,6 +401,19 @@ void orc_matvec(int64_t n, const int* ro, const int* cols, const double* vals,
   }
 }
 
+void orc_matvec_f32(int64_t n, const int* ro, const int* cols, const double* vals,
+                    const double* x, double* y) {
+  for (int64_t row = 0; row < n; ++row) {
+    float sum = 0.0f;
+    for (int i = ro[row]; i < ro[row + 1]; ++i) {
+      const float a = (float)vals[i], b = (float)x[cols[i]];
+      const float p = a * b;
+      sum += p;
+    }
+    y[row] = (double)sum;
+  }
+}
+
 double orc_dot(int64_t n, const double* x, const double* y) {
   double r = 0;
   for (int64_t i = 0; i < n; ++i) r += x[i] * y[i];

@@ -85,6 +85,11 @@ void orc_impose_dirichlet(double value, int64_t nrows, const int* row_offsets,
 /* ref/src/SparseMatrix_functions.hpp:545-578 (y initial sum = 0*y[row]). */
 void orc_matvec(int64_t nrows, const int* row_offsets, const int* cols,
                 const double* vals, const double* x, double* y);
+/* The mixed-precision solve's inner SpMV (no reference counterpart): values and
+ * x rounded to float, float products summed in stored column order from +0.0,
+ * y widened to double.  Pinned by the FP64 matvec it rounds (error bound). */
+void orc_matvec_f32(int64_t nrows, const int* row_offsets, const int* cols,
+                    const double* vals, const double* x, double* y);
 /* ref/src/Vector_functions.hpp:194-226 */
 double orc_dot(int64_t n, const double* x, const double* y);
 /* ref/src/Vector_functions.hpp:127-150 */

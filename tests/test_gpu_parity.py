@@ -219,6 +219,49 @@ def test_long_rows_take_the_rowwise_path(mfe, ctx, oracle_mod):
     assert bits_equal(y.to_numpy(), oracle_mod.matvec(Ao, xh))
 
 
+@pytest.mark.parametrize("dims", [(10, 10, 10), (40, 33, 29)])
+def test_matvec_fp32_bit_exact(mfe, ctx, oracle_mod, dims):
+    """The mixed solve's FP32 SpMV equals the oracle's FP32 restatement bit for bit."""
+    say(f"case matvec fp32 {dims}")
+    mesh, A, b, _ = build_stages(mfe, ctx, dims)
+    P = oracle_mod.build_problem(*dims)
+    xh = np.random.default_rng(4).standard_normal(A.nrows)
+    x = mfe.Vector.from_numpy(xh, ctx)
+    y = mfe.Vector(A.nrows, ctx)
+    mfe.matvec_fp32(A, x, y)
+    assert bits_equal(y.to_numpy(), oracle_mod.matvec_f32(P["A"], xh))
+    # the FP32 copy follows later changes of the FP64 values
+    mfe.impose_dirichlet(0.0, A, b, dims[0] + 1, dims[1] + 1, dims[2] + 1, [5, 6, 7])
+    Ao = P["A"].copy()
+    oracle_mod.impose_dirichlet(0.0, Ao, P["b"].copy(), np.array([5, 6, 7], np.int32))
+    mfe.matvec_fp32(A, x, y)
+    assert bits_equal(y.to_numpy(), oracle_mod.matvec_f32(Ao, xh))
+
+
+@pytest.mark.parametrize("dims", [(30, 30, 30), (64, 48, 40)])
+def test_mixed_precision_refinement(mfe, ctx, dims):
+    """FP32 inner CG + FP64 refinement reaches ||b - A x|| <= 1e-10 ||b||
+    (recomputed here in FP64) and agrees with the FP64 CG solution."""
+    say(f"case mixed {dims}")
+    mesh, A, b, _ = build_stages(mfe, ctx, dims)
+    x = mfe.Vector(A.nrows, ctx)
+    r = mfe.cg_solve_mixed(A, b, x, inner_tol=1e-5, outer_tol=1e-10)
+    assert r.converged and r.outer_iters >= 1 and r.rel_residual <= 1e-10
+    y = mfe.Vector(A.nrows, ctx)
+    mfe.matvec(A, x, y)
+    bh = b.to_numpy()
+    true_rel = np.linalg.norm(bh - y.to_numpy()) / np.linalg.norm(bh)
+    assert true_rel <= 1e-10
+    x64 = mfe.Vector(A.nrows, ctx)
+    r64 = mfe.cg_solve(A, b, x64, max_iter=5000, tolerance=1e-10, relative=True)
+    xm, xf = x.to_numpy(), x64.to_numpy()
+    err = np.linalg.norm(xm - xf) / np.linalg.norm(xf)
+    say(f"mixed: outer {r.outer_iters} inner {r.inner_iters} rel {r.rel_residual:.2e} "
+        f"(fp64 cg {r64.num_iters} its); |x_mixed - x_fp64|/|x_fp64| = {err:.2e}")
+    assert err <= 1e-7
+    assert mfe.verify_solution(mesh, x)[0] == 0
+
+
 def test_upload_rejects_bad_columns(mfe, ctx):
     with pytest.raises(mfe.MfeError) as e:
         mfe.CSRMatrix.from_host([0, 1, 2], [0, 5], [1.0, 1.0], ctx=ctx)

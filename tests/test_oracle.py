@@ -264,6 +264,27 @@ def test_matvec_dot_waxpby_definitions(oracle_mod):
     assert bits_equal(w, 0.5 * x + (-2.0) * y)
 
 
+def test_matvec_f32_pins(oracle_mod):
+    """FP32 SpMV: exact wherever float arithmetic is exact (small integers),
+    and within the FP32 error bound of the FP64 product elsewhere."""
+    P = oracle_mod.build_problem(7, 6, 5)
+    A = P["A"].copy()
+    rng = np.random.default_rng(5)
+    A.vals = rng.integers(-8, 9, A.nnz).astype(np.float64)
+    x = rng.integers(-16, 17, A.nrows).astype(np.float64)
+    assert np.array_equal(oracle_mod.matvec_f32(A, x), oracle_mod.matvec(A, x))
+    A = P["A"]
+    x = rng.standard_normal(A.nrows)
+    y64 = oracle_mod.matvec(A, x)
+    y32 = oracle_mod.matvec_f32(A, x)
+    eps32 = float(np.finfo(np.float32).eps)
+    absum = oracle_mod.matvec(oracle_mod.CSR(A.rows, A.row_offsets, A.cols, np.abs(A.vals)),
+                              np.abs(x))
+    lens = np.diff(A.row_offsets)
+    assert np.all(np.abs(y32 - y64) <= (lens + 3) * eps32 * absum + 1e-300)
+    assert not np.array_equal(y32, y64)  # it really is FP32
+
+
 # ---------------------------------------------------------------------- CG --
 
 @pytest.mark.parametrize("dims", SMALL_MESHES)
