@@ -146,7 +146,9 @@ int mfe_waxpby(mfe_ctx* ctx, double alpha, const mfe_vec* x, double beta, const 
  * previous iteration exceeds tolerance (the reference's loop test), so
  * num_iters and normr mean what they mean there.  Returns MFE_OK or
  * MFE_BREAKDOWN (num_iters = last completed k).  times (nullable) receives
- * seconds in the MFE_T_* slots; history (nullable) receives 4 doubles per
+ * seconds in the MFE_T_* slots: TOTAL always; WAXPY, DOT and MATVEC only with
+ * MFE_CG_EAGER (GPU time between kernel groups; a HIP graph launch cannot be
+ * split, so graph mode leaves them 0).  history (nullable) receives 4 doubles per
  * iteration k (entry k-1): {rtrans, p_ap_dot, alpha, beta}.
  * flags: MFE_CG_REPLAY reproduces the reference's sequential dot products and
  * hence its scalar trajectory bit for bit; the default fast mode uses tree

@@ -47,8 +47,26 @@ def _compile(src: str) -> str:
     return obj
 
 
+DRIVER_SRC = os.path.join(HERE, "host", "miniFE_hip_main.cpp")
+DRIVER = os.path.join(HERE, "miniFE_hip.x")
+
+
+def build_driver() -> str:
+    """The miniFE.x-compatible driver (host C++ over the C-ABI)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(HERE, "host"), DRIVER_SRC, "-o", DRIVER + ".tmp", "-L", HERE,
+           "-lminife_hip", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+           "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 def build(verbose: bool = False) -> str:
-    """Compile all sources and link the shared library; returns its path."""
+    """Compile all sources and link the shared library (and the driver);
+    returns the library path."""
     os.makedirs(OBJDIR, exist_ok=True)
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(_compile, SOURCES))
@@ -58,8 +76,9 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_driver()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", DRIVER)
     return LIB
 
 

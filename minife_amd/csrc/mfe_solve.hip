@@ -543,6 +543,52 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(int64_t n, const CgSt
 
 // --------------------------------------------------------- workspace -----
 
+// Attribution of GPU time to the reference's CG timer slots (WAXPY, DOT,
+// MATVEC; mytimer.hpp:36-43) with events recorded between kernel groups on
+// the solve's stream.  Eager mode only: events cannot split a graph launch.
+struct OpTimer {
+  std::vector<hipEvent_t> ev;
+  std::vector<int> cat;  // slot of the interval that ends at ev[i]
+  size_t used = 0;
+  hipStream_t s = nullptr;
+  bool on = false;
+  ~OpTimer() {
+    for (hipEvent_t e : ev) hipEventDestroy(e);
+  }
+  bool grow(size_t need) {
+    while (ev.size() < need) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return false;
+      ev.push_back(e);
+      cat.push_back(-1);
+    }
+    return true;
+  }
+  void begin(hipStream_t st) {
+    s = st;
+    used = 0;
+    on = grow(1) && hipEventRecord(ev[0], s) == hipSuccess;
+    used = on ? 1 : 0;
+  }
+  void mark(int slot) {
+    if (!on) return;
+    if (!grow(used + 1) || hipEventRecord(ev[used], s) != hipSuccess) {
+      on = false;
+      return;
+    }
+    cat[used++] = slot;
+  }
+  void accumulate(double* times) {
+    if (!on || used < 2 || hipEventSynchronize(ev[used - 1]) != hipSuccess) return;
+    for (size_t i = 1; i < used; ++i) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, ev[i - 1], ev[i]) == hipSuccess && cat[i] >= 0)
+        times[cat[i]] += 1e-3 * ms;
+    }
+    on = false;
+  }
+};
+
 template <typename T>
 struct VecSet {
   int64_t n = -1;
@@ -584,6 +630,7 @@ struct CgWorkspace {
   double* hist = nullptr;
   int64_t hist_cap = 0;
   GraphSlot g64, g32;
+  OpTimer timer;
 };
 
 constexpr int kGraphChunk = 25;  // CG iterations per captured graph
@@ -646,33 +693,46 @@ static unsigned vec_grid(int64_t n) {
   return (unsigned)(b < kVecBlocks ? (b > 0 ? b : 1) : kVecBlocks);
 }
 
+static inline void mark(OpTimer* tm, int slot) {
+  if (tm) tm->mark(slot);
+}
+
 // One CG iteration body (k = 1 when y_for_p == r, k >= 2 when y_for_p == p).
 template <typename T>
 static hipError_t enqueue_iteration(const mfe_csr* A, const T* vals, VecSet<T>& v, CgState* st,
                                     double* hist, T* x, const T* y_for_p, bool replay,
-                                    hipStream_t s) {
+                                    hipStream_t s, OpTimer* tm = nullptr) {
   const int64_t n = v.n;
   const unsigned vg = vec_grid(n);
   cg_p_kernel<T><<<vg, kBlock, 0, s>>>(n, st, v.r, y_for_p, v.p, x);
+  mark(tm, MFE_T_WAXPY);
   int tiles = 0;
   const int* done = &st->done;
   if constexpr (std::is_same<T, double>::value) {
     if (replay) {
       launch_spmv<double, true, kPlain>(A, vals, v.p, v.Ap, nullptr, nullptr, done, s, &tiles);
+      mark(tm, MFE_T_MATVEC);
       cg_alpha_seq_kernel<<<1, kBlock, 0, s>>>(st, n, v.Ap, v.p, hist);
+      mark(tm, MFE_T_DOT);
       cg_update_kernel<double, false><<<vg, kBlock, 0, s>>>(n, st, v.r, v.Ap, nullptr);
+      mark(tm, MFE_T_WAXPY);
       cg_next_seq_kernel<<<1, kBlock, 0, s>>>(st, n, v.r);
+      mark(tm, MFE_T_DOT);
       return hipGetLastError();
     }
   }
   launch_spmv<T, false, kDot>(A, vals, v.p, v.Ap, nullptr, v.partials, done, s, &tiles);
+  mark(tm, MFE_T_MATVEC);
   const int nb2 = (int)cdiv(tiles, kStage);
   reduce_stage_kernel<T><<<nb2, kBlock, 0, s>>>(v.partials, tiles, 1, v.red2, done);
   cg_alpha_fast_kernel<T><<<1, kBlock, 0, s>>>(st, v.red2, nb2, hist);
   cg_norms_kernel<T><<<vg, kBlock, 0, s>>>(n, st, v.Ap, v.p, v.partials);
   cg_norms_fin_kernel<T><<<1, kBlock, 0, s>>>(st, v.partials, (int)vg, hist);
+  mark(tm, MFE_T_DOT);
   cg_update_kernel<T, true><<<vg, kBlock, 0, s>>>(n, st, v.r, v.Ap, v.partials);
+  mark(tm, MFE_T_WAXPY);
   cg_next_fast_kernel<T><<<1, kBlock, 0, s>>>(st, v.partials, (int)vg);
+  mark(tm, MFE_T_DOT);
   return hipGetLastError();
 }
 
@@ -681,20 +741,26 @@ static hipError_t enqueue_iteration(const mfe_csr* A, const T* vals, VecSet<T>& 
 template <typename T>
 static int run_cg(mfe_ctx* ctx, const mfe_csr* A, const T* vals, const T* b, T* x, int max_iter,
                   double tol, bool replay, bool eager, CgWorkspace* w, VecSet<T>& v,
-                  GraphSlot& slot, const void* key_b, const void* key_x, unsigned key_flags) {
+                  GraphSlot& slot, const void* key_b, const void* key_x, unsigned key_flags,
+                  double* times = nullptr) {
   hipStream_t s = ctx->stream;
   const int64_t n = v.n;
+  OpTimer* tm = (eager && times) ? &w->timer : nullptr;
+  if (tm) tm->begin(s);
   cg_init_kernel<<<1, 1, 0, s>>>(w->st, max_iter, tol);
   MFE_LAUNCHED(ctx, "cg_init_kernel");
   if (n > 0) {
     const unsigned vg = vec_grid(n);
     // p = 1*x + 0*x ; Ap = A p ; r = 1*b + (-1)*Ap ; rtrans = r.r  (cg_solve.hpp:124-136)
     waxpby_kernel<T><<<vg, kBlock, 0, s>>>(n, T(1), x, T(0), x, v.p);
+    mark(tm, MFE_T_WAXPY);
     // The reference's Ap starts as a zero vector, so its 0*Ap[row] is +0.0:
     // start the sums at +0.0 instead of reading the workspace.
     int e = launch_spmv<T, false, kPlain>(A, vals, v.p, v.Ap, nullptr, nullptr, nullptr, s, nullptr);
     if (e) return hip_fail(ctx, (hipError_t)e, "cg prologue spmv");
+    mark(tm, MFE_T_MATVEC);
     waxpby_kernel<T><<<vg, kBlock, 0, s>>>(n, T(1), b, T(-1), v.Ap, v.r);
+    mark(tm, MFE_T_WAXPY);
     bool seq = false;
     if constexpr (std::is_same<T, double>::value) seq = replay;
     if (seq) {
@@ -704,14 +770,15 @@ static int run_cg(mfe_ctx* ctx, const mfe_csr* A, const T* vals, const T* b, T* 
       dot_partial_kernel<T><<<vg, kBlock, 0, s>>>(n, v.r, v.r, v.partials);
       cg_first_fast_kernel<T><<<1, kBlock, 0, s>>>(w->st, v.partials, (int)vg);
     }
+    mark(tm, MFE_T_DOT);
     MFE_LAUNCHED(ctx, "cg prologue");
     // Iteration 1: p = 1*r + 0*r.
-    hipError_t he = enqueue_iteration<T>(A, vals, v, w->st, w->hist, x, v.r, replay, s);
+    hipError_t he = enqueue_iteration<T>(A, vals, v, w->st, w->hist, x, v.r, replay, s, tm);
     if (he != hipSuccess) return hip_fail(ctx, he, "cg iteration 1");
     if (max_iter > 1) {
       if (eager) {
         for (int k = 2; k <= max_iter; ++k) {
-          he = enqueue_iteration<T>(A, vals, v, w->st, w->hist, x, v.p, replay, s);
+          he = enqueue_iteration<T>(A, vals, v, w->st, w->hist, x, v.p, replay, s, tm);
           if (he != hipSuccess) return hip_fail(ctx, he, "cg iteration");
           if (k % kGraphChunk == 0) {
             MFE_CHECK(ctx, hipMemcpyAsync(w->h_st, w->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
@@ -755,10 +822,12 @@ static int run_cg(mfe_ctx* ctx, const mfe_csr* A, const T* vals, const T* b, T* 
       }
     }
     cg_finish_kernel<T><<<vg, kBlock, 0, s>>>(n, w->st, v.p, x);
+    mark(tm, MFE_T_WAXPY);
     MFE_LAUNCHED(ctx, "cg_finish_kernel");
   }
   MFE_CHECK(ctx, hipMemcpyAsync(w->h_st, w->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
   MFE_CHECK(ctx, hipStreamSynchronize(s));
+  if (tm) tm->accumulate(times);
   return MFE_OK;
 }
 
@@ -965,9 +1034,10 @@ int mfe_cg_solve(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x, i
   if ((st = ws_get(ctx, max_iter, &w)) || (st = vecset_prepare<double>(ctx, w->v64, n, w->g64)))
     return st;
   *num_iters = 0;
+  double op_times[5] = {0, 0, 0, 0, 0};
   MFE_CHECK(ctx, hipEventRecord(ctx->t_start, ctx->stream));
   st = run_cg<double>(ctx, A, A->vals, b->d, x->d, max_iter, tol, replay, eager, w, w->v64, w->g64,
-                      b, x, flags & MFE_CG_REPLAY);
+                      b, x, flags & MFE_CG_REPLAY, times ? op_times : nullptr);
   if (st) return st;
   MFE_CHECK(ctx, hipEventRecord(ctx->t_stop, ctx->stream));
   MFE_CHECK(ctx, hipEventSynchronize(ctx->t_stop));
@@ -977,7 +1047,7 @@ int mfe_cg_solve(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x, i
   if (times) {
     float ms = 0;
     MFE_CHECK(ctx, hipEventElapsedTime(&ms, ctx->t_start, ctx->t_stop));
-    for (int i = 0; i < 5; ++i) times[i] = 0;
+    for (int i = 0; i < 5; ++i) times[i] = op_times[i];
     times[MFE_T_TOTAL] = 1e-3 * ms;
   }
   const int rows = hs.num_iters + (hs.status == MFE_BREAKDOWN ? 1 : 0);

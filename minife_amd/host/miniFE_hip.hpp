@@ -1,0 +1,316 @@
+// miniFE_hip.hpp -- the reference's ref/ operator surface on MI355X.
+//
+// A header-only C++ layer with the names, signatures, argument meaning and
+// error behaviour of miniFE's ref/ operators (namespace miniFE), whose bodies
+// call the C-ABI of libminife_hip.so (include/minife_hip.h).  Matrices and
+// vectors live in HBM; host copies are made only on request (coefs(),
+// download()).  Scalar = double and ordinals = int, as ref/src/Makefile.gnu.serial.
+//
+//   reference (ref/)                                   here
+//   Box                       utils/Box.hpp:41-53      miniFE::Box
+//   Parameters                utils/Parameters.hpp     miniFE::Parameters
+//   simple_mesh_description   src/simple_mesh_description.hpp:38-244
+//   CSRMatrix<S,LO,GO>        src/CSRMatrix.hpp:40-129 (device-resident)
+//   Vector<S,LO,GO>           src/Vector.hpp:36-61     (device-resident)
+//   generate_matrix_structure src/generate_matrix_structure.hpp:47-149
+//   assemble_FE_data          src/assemble_FE_data.hpp:38-73
+//   perform_element_loop      src/perform_element_loop.hpp:40-104
+//   impose_dirichlet          src/SparseMatrix_functions.hpp:431-480
+//   make_local_matrix         src/make_local_matrix.hpp:42-442
+//   matvec_std / matvec_and_dot src/SparseMatrix_functions.hpp:491-578
+//   dot / waxpby              src/Vector_functions.hpp:127-226
+//   cg_solve                  src/cg_solve.hpp:69-229
+//   verify_solution           fem/verify_solution.hpp:55-174
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <limits>
+#include <set>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "minife_hip.h"
+
+namespace miniFE {
+
+typedef double timer_type;
+enum CG_TIMES { WAXPY = 0, DOT = 1, MATVEC = 2, MATVECDOT = 3, TOTAL = 4, NUM_TIMERS = 5 };
+
+// Runtime options of the HIP path that have no reference counterpart.
+struct HipOptions {
+  bool replay = false;         // cg_solve: the reference's sequential dots (bit-exact trajectory)
+  bool eager = false;          // cg_solve: launch kernel by kernel (fills WAXPY/DOT/MATVEC)
+  bool relative_tol = false;   // cg_solve: stop on ||r|| <= tol * ||b||
+};
+
+inline HipOptions& hip_options() {
+  static HipOptions o;
+  return o;
+}
+
+// The device context every object below uses (device from MINIFE_DEVICE, default 0).
+inline mfe_ctx* context() {
+  static mfe_ctx* ctx = [] {
+    mfe_ctx* c = nullptr;
+    const char* d = std::getenv("MINIFE_DEVICE");
+    int st = mfe_ctx_create(d ? std::atoi(d) : 0, &c);
+    if (st != MFE_OK) throw std::runtime_error("miniFE: cannot create a HIP context (no GPU?)");
+    return c;
+  }();
+  return ctx;
+}
+
+inline void check(int st, const char* what) {
+  if (st != MFE_OK)
+    throw std::runtime_error(std::string("miniFE::") + what + ": " + mfe_last_error(context()));
+}
+
+struct Box {
+  int ranges[6];
+  int* operator[](int xyz) { return &ranges[xyz * 2]; }
+  const int* operator[](int xyz) const { return &ranges[xyz * 2]; }
+};
+
+struct Parameters {
+  int nx = 10, ny = 10, nz = 10, numthreads = 1, mv_overlap_comm_comp = 0, use_locking = 0;
+  float load_imbalance = 0;
+  std::string name;
+  int elem_group_size = 1, use_elem_mat_fields = 1, verify_solution = 0;
+  int device = 0, num_devices = 2, skip_device = 9999, numa = 1;
+};
+
+template <typename GlobalOrdinal>
+class simple_mesh_description {
+ public:
+  // One rank: local_box must equal global_box.  bc_rows_0 holds the nodes of
+  // the faces x=0, y=0, z=0, y=1, z=1 and bc_rows_1 those of x=1
+  // (simple_mesh_description.hpp:77-224); node id == row.
+  simple_mesh_description(const Box& global_box_in, const Box& local_box_in) {
+    static_assert(std::is_same<GlobalOrdinal, int>::value, "GlobalOrdinal must be int");
+    for (int i = 0; i < 6; ++i) {
+      global_box.ranges[i] = global_box_in.ranges[i];
+      local_box.ranges[i] = local_box_in.ranges[i];
+      if (global_box.ranges[i] != local_box.ranges[i])
+        throw std::runtime_error("miniFE (HIP): one rank per GPU; local_box must equal global_box");
+    }
+    const int gx = global_box[0][1] + 1, gy = global_box[1][1] + 1, gz = global_box[2][1] + 1;
+    for (int iz = 0; iz < gz; ++iz)
+      for (int iy = 0; iy < gy; ++iy)
+        for (int ix = 0; ix < gx; ++ix) {
+          const GlobalOrdinal id = ix + gx * iy + gx * gy * iz;
+          if (ix == 0 || iy == 0 || iz == 0 || iy == gy - 1 || iz == gz - 1) bc_rows_0.insert(id);
+          if (ix == gx - 1) bc_rows_1.insert(id);
+        }
+  }
+  GlobalOrdinal map_id_to_row(const GlobalOrdinal& id) const { return id; }
+  GlobalOrdinal max_row_in_map() const { return 0; }
+
+  std::set<GlobalOrdinal> bc_rows_0;
+  std::set<GlobalOrdinal> bc_rows_1;
+  Box global_box;
+  Box local_box;
+};
+
+template <typename Scalar, typename LocalOrdinal, typename GlobalOrdinal>
+struct CSRMatrix {
+  static_assert(std::is_same<Scalar, double>::value && std::is_same<LocalOrdinal, int>::value &&
+                    std::is_same<GlobalOrdinal, int>::value,
+                "the HIP path is built for Scalar=double, LocalOrdinal=GlobalOrdinal=int");
+  typedef Scalar ScalarType;
+  typedef LocalOrdinal LocalOrdinalType;
+  typedef GlobalOrdinal GlobalOrdinalType;
+
+  CSRMatrix() { check(mfe_csr_create(context(), &h), "CSRMatrix"); }
+  ~CSRMatrix() { mfe_csr_destroy(h); }
+  CSRMatrix(const CSRMatrix&) = delete;
+  CSRMatrix& operator=(const CSRMatrix&) = delete;
+
+  size_t num_rows() const {
+    int64_t n = 0;
+    mfe_csr_shape(h, &n, nullptr);
+    return (size_t)n;
+  }
+  size_t num_nonzeros() const {
+    int64_t z = 0;
+    mfe_csr_shape(h, nullptr, &z);
+    return (size_t)z;
+  }
+  // Host copies of the device arrays (CSRMatrix.hpp:64-70 fields).
+  void download(std::vector<GlobalOrdinal>* rows, std::vector<LocalOrdinal>* row_offsets,
+                std::vector<GlobalOrdinal>* packed_cols, std::vector<Scalar>* packed_coefs) const {
+    const size_t n = num_rows(), z = num_nonzeros();
+    if (rows) rows->resize(n);
+    if (row_offsets) row_offsets->resize(n + 1);
+    if (packed_cols) packed_cols->resize(z);
+    if (packed_coefs) packed_coefs->resize(z);
+    check(mfe_csr_download(context(), h, rows ? rows->data() : nullptr,
+                           row_offsets ? row_offsets->data() : nullptr,
+                           packed_cols ? packed_cols->data() : nullptr,
+                           packed_coefs ? packed_coefs->data() : nullptr),
+          "CSRMatrix::download");
+  }
+
+  bool has_local_indices = false;
+  mfe_csr* h = nullptr;
+};
+
+template <typename Scalar, typename LocalOrdinal, typename GlobalOrdinal>
+struct Vector {
+  typedef Scalar ScalarType;
+  typedef LocalOrdinal LocalOrdinalType;
+  typedef GlobalOrdinal GlobalOrdinalType;
+
+  // Vector.hpp:42-50: zero-filled.
+  Vector(GlobalOrdinal startIdx, LocalOrdinal local_sz) : startIndex(startIdx), local_size(local_sz) {
+    check(mfe_vec_create(context(), local_sz, &h), "Vector");
+  }
+  ~Vector() { mfe_vec_destroy(h); }
+  Vector(const Vector&) = delete;
+  Vector& operator=(const Vector&) = delete;
+
+  std::vector<Scalar> coefs() const {
+    std::vector<Scalar> out((size_t)local_size);
+    check(mfe_vec_download(context(), h, out.data(), local_size), "Vector::coefs");
+    return out;
+  }
+  void assign(const std::vector<Scalar>& v) {
+    check(mfe_vec_upload(context(), h, v.data(), (int64_t)v.size()), "Vector::assign");
+  }
+
+  GlobalOrdinal startIndex;
+  LocalOrdinal local_size;
+  mfe_vec* h = nullptr;
+};
+
+// ---- assembly ---------------------------------------------------------------
+
+template <typename MatrixType>
+int generate_matrix_structure(
+    const simple_mesh_description<typename MatrixType::GlobalOrdinalType>& mesh, MatrixType& A) {
+  const int st = mfe_generate_matrix_structure(context(), mesh.global_box.ranges,
+                                               mesh.local_box.ranges, A.h);
+  if (st == MFE_OOM) return 1;  // generate_matrix_structure.hpp:140-148
+  check(st, "generate_matrix_structure");
+  A.has_local_indices = false;
+  return 0;
+}
+
+template <typename MatrixType, typename VectorType>
+void perform_element_loop(
+    const simple_mesh_description<typename MatrixType::GlobalOrdinalType>& mesh,
+    const Box& local_elem_box, MatrixType& A, VectorType& b, Parameters& /*params*/) {
+  check(mfe_perform_element_loop(context(), mesh.global_box.ranges, local_elem_box.ranges, A.h,
+                                 b.h, nullptr),
+        "perform_element_loop");
+}
+
+template <typename MatrixType, typename VectorType>
+void assemble_FE_data(const simple_mesh_description<typename MatrixType::GlobalOrdinalType>& mesh,
+                      MatrixType& A, VectorType& b, Parameters& /*params*/) {
+  check(mfe_assemble_FE_data(context(), mesh.global_box.ranges, mesh.local_box.ranges, A.h, b.h,
+                             nullptr),
+        "assemble_FE_data");
+}
+
+template <typename MatrixType, typename VectorType>
+void impose_dirichlet(typename MatrixType::ScalarType prescribed_value, MatrixType& A,
+                      VectorType& b, int global_nx, int global_ny, int global_nz,
+                      const std::set<typename MatrixType::GlobalOrdinalType>& bc_rows) {
+  std::vector<int> rows(bc_rows.begin(), bc_rows.end());  // ascending, unique
+  check(mfe_impose_dirichlet(context(), prescribed_value, A.h, b.h, global_nx, global_ny,
+                             global_nz, rows.data(), (int64_t)rows.size()),
+        "impose_dirichlet");
+}
+
+template <typename MatrixType>
+void make_local_matrix(MatrixType& A) {
+  check(mfe_make_local_matrix(context(), A.h), "make_local_matrix");
+  A.has_local_indices = true;
+}
+
+// ---- solve --------------------------------------------------------------------
+
+template <typename MatrixType, typename VectorType>
+struct matvec_std {
+  void operator()(MatrixType& A, VectorType& x, VectorType& y) {
+    check(mfe_matvec(context(), A.h, x.h, y.h), "matvec_std");
+  }
+};
+
+template <typename MatrixType, typename VectorType>
+void matvec(MatrixType& A, VectorType& x, VectorType& y) {
+  matvec_std<MatrixType, VectorType>()(A, x, y);
+}
+
+template <typename MatrixType, typename VectorType>
+typename VectorType::ScalarType matvec_and_dot(MatrixType& A, VectorType& x, VectorType& y) {
+  double r = 0;
+  check(mfe_matvec_and_dot(context(), A.h, x.h, y.h, &r), "matvec_and_dot");
+  return r;
+}
+
+template <typename VectorType>
+typename VectorType::ScalarType dot(const VectorType& x, const VectorType& y) {
+  double r = 0;
+  check(hip_options().replay ? mfe_dot_replay(context(), x.h, y.h, &r)
+                             : mfe_dot(context(), x.h, y.h, &r),
+        "dot");
+  return r;
+}
+
+template <typename VectorType>
+void waxpby(typename VectorType::ScalarType alpha, const VectorType& x,
+            typename VectorType::ScalarType beta, const VectorType& y, VectorType& w) {
+  check(mfe_waxpby(context(), alpha, x.h, beta, y.h, w.h), "waxpby");
+}
+
+// cg_solve.hpp:69-229.  Matvec is accepted for signature compatibility; the
+// solver always runs the library's fused CSR kernels.  On breakdown it prints
+// the reference's message and returns with num_iters = the last completed k.
+template <typename OperatorType, typename VectorType, typename Matvec>
+void cg_solve(OperatorType& A, const VectorType& b, VectorType& x, Matvec /*matvec*/,
+              typename OperatorType::LocalOrdinalType max_iter,
+              typename OperatorType::ScalarType& tolerance,
+              typename OperatorType::LocalOrdinalType& num_iters,
+              typename OperatorType::ScalarType& normr, timer_type* my_cg_times) {
+  const HipOptions& o = hip_options();
+  unsigned flags = (o.replay ? MFE_CG_REPLAY : 0u) | (o.eager ? MFE_CG_EAGER : 0u) |
+                   (o.relative_tol ? MFE_CG_TOL_RELATIVE : 0u);
+  double times[NUM_TIMERS] = {0, 0, 0, 0, 0};
+  int ni = 0;
+  double nr = 0;
+  const int st = mfe_cg_solve(context(), A.h, b.h, x.h, max_iter, tolerance, flags, &ni, &nr,
+                              times, nullptr);
+  if (st == MFE_NOT_LOCAL) {
+    std::fprintf(stderr, "miniFE::cg_solve ERROR, A.has_local_indices is false, needs to be "
+                         "true. This probably means miniFE::make_local_matrix(A) was not called "
+                         "prior to calling miniFE::cg_solve.\n");
+    return;
+  }
+  if (st == MFE_BREAKDOWN) std::fprintf(stderr, "miniFE::cg_solve ERROR, numerical breakdown!\n");
+  else check(st, "cg_solve");
+  num_iters = ni;
+  normr = nr;
+  if (my_cg_times)
+    for (int i = 0; i < NUM_TIMERS; ++i) my_cg_times[i] = times[i];
+}
+
+// verify_solution.hpp:55-174; 0 when within tolerance.
+template <typename VectorType>
+int verify_solution(const simple_mesh_description<typename VectorType::GlobalOrdinalType>& mesh,
+                    const VectorType& x, double tolerance, bool verify_whole_domain = false,
+                    double* max_err = nullptr) {
+  double e = 0;
+  const int rc = mfe_verify_solution(context(), mesh.global_box.ranges, x.h, tolerance,
+                                     verify_whole_domain ? 1 : 0, &e);
+  if (rc != 0 && rc != 1) check(rc, "verify_solution");
+  if (max_err) *max_err = e;
+  return rc;
+}
+
+}  // namespace miniFE

@@ -1,0 +1,303 @@
+// miniFE_hip.x -- the reference's miniFE.x driver on MI355X.
+//
+// Restates ref/src/main.cpp:86-188 and ref/src/driver.hpp:120-404 over the
+// miniFE_hip.hpp operator layer: same key=value parameters, same phases and
+// console lines, same YAML report keys (driver.hpp:314-400, including the
+// reference's flop accounting), written to miniFE.NXxNYxNZ.[name.]DATE.yaml.
+//
+// Extra keys (no reference counterpart): max_iters (default 200), tolerance
+// (default DBL_EPSILON), relative_tol=1, replay=1 (reference-exact CG
+// trajectory), eager=1 (per-kernel CG timers), mixed=1 (FP32-inner / FP64
+// refinement solve to outer_tol, default 1e-10).
+#include <sys/time.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <iostream>
+#include <limits>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "miniFE_hip.hpp"
+
+namespace {
+
+double mytimer() {  // ref/utils/mytimer.cpp:106-127 (gettimeofday)
+  struct timeval tv;
+  gettimeofday(&tv, nullptr);
+  return (double)tv.tv_sec + tv.tv_usec / 1000000.0;
+}
+
+// "key: value" tree printed with two-space indentation, as YAML_Element.
+struct Yaml {
+  std::string key, value;
+  std::vector<std::unique_ptr<Yaml>> kids;
+  Yaml* add(const std::string& k, const std::string& v = "") {
+    value = "";
+    kids.emplace_back(new Yaml{k, v, {}});
+    return kids.back().get();
+  }
+  template <typename T>
+  Yaml* add(const std::string& k, T v) {
+    std::ostringstream s;
+    s << v;
+    return add(k, s.str());
+  }
+  Yaml* get(const std::string& k) {
+    for (auto& c : kids)
+      if (c->key == k) return c.get();
+    return nullptr;
+  }
+  std::string print(const std::string& space) const {
+    std::string s = space + key + ": " + value + "\n";
+    for (auto& c : kids) s += c->print(space + "  ");
+    return s;
+  }
+};
+
+// Parameter lookup as Mantevo::parse_parameter: "key=value", "key: value" or
+// "key value" anywhere in the arguments.
+std::string arg_value(int argc, char** argv, const std::string& key) {
+  for (int i = 1; i < argc; ++i) {
+    std::string a = argv[i];
+    if (a.rfind(key + "=", 0) == 0) return a.substr(key.size() + 1);
+    if ((a == key || a == key + ":") && i + 1 < argc) return argv[i + 1];
+  }
+  return "";
+}
+template <typename T>
+T param(int argc, char** argv, const std::string& key, T dflt) {
+  std::string v = arg_value(argc, argv, key);
+  if (v.empty()) return dflt;
+  std::istringstream s(v);
+  T out = dflt;
+  s >> out;
+  return out;
+}
+
+#define RUN_TIMED_FUNCTION(msg, fn, time_inc, time_total)          \
+  {                                                                \
+    std::cout.width(30);                                           \
+    std::cout << msg;                                              \
+    std::cout.flush();                                             \
+    double rtf_t0 = mytimer();                                     \
+    fn;                                                            \
+    time_inc = mytimer() - rtf_t0;                                 \
+    time_total += time_inc;                                        \
+    std::cout << time_inc << "s, total time: " << time_total << std::endl; \
+  }
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  using namespace miniFE;
+  typedef CSRMatrix<double, int, int> MatrixType;
+  typedef Vector<double, int, int> VectorType;
+
+  Parameters params;
+  params.nx = param(argc, argv, "nx", 10);
+  params.ny = param(argc, argv, "ny", params.nx);
+  params.nz = param(argc, argv, "nz", params.ny);
+  params.verify_solution = param(argc, argv, "verify_solution", 0);
+  params.name = arg_value(argc, argv, "name");
+  const int max_iters = param(argc, argv, "max_iters", 200);
+  double tol = param(argc, argv, "tolerance", std::numeric_limits<double>::epsilon());
+  HipOptions& opt = hip_options();
+  opt.replay = param(argc, argv, "replay", 0) != 0;
+  opt.eager = param(argc, argv, "eager", 0) != 0;
+  opt.relative_tol = param(argc, argv, "relative_tol", 0) != 0;
+  const bool mixed = param(argc, argv, "mixed", 0) != 0;
+  const double outer_tol = param(argc, argv, "outer_tol", 1e-10);
+
+  const double start_time = mytimer();
+  Box global_box = {{0, params.nx, 0, params.ny, 0, params.nz}};
+  Box my_box = global_box;
+
+  Yaml doc;
+  doc.add("Global Run Parameters");
+  doc.get("Global Run Parameters")->add("dimensions");
+  doc.get("Global Run Parameters")->get("dimensions")->add("nx", params.nx);
+  doc.get("Global Run Parameters")->get("dimensions")->add("ny", params.ny);
+  doc.get("Global Run Parameters")->get("dimensions")->add("nz", params.nz);
+  doc.get("Global Run Parameters")->add("load_imbalance", params.load_imbalance);
+  doc.get("Global Run Parameters")->add("mv_overlap_comm_comp", "0 (no)");
+  doc.get("Global Run Parameters")->add("number of processors", 1);
+  char devname[256] = {0};
+  int cus = 0;
+  int64_t hbm = 0;
+  mfe_device_info(context(), devname, sizeof devname, &cus, &hbm);
+  doc.add("Platform");
+  doc.get("Platform")->add("device", std::string(devname));
+  doc.get("Platform")->add("compute units", cus);
+  doc.get("Platform")->add("HBM (GB)", hbm / 1e9);
+  doc.add("Build");
+  doc.get("Build")->add("using MPI", "no");
+  doc.get("Build")->add("library", "libminife_hip.so (gfx950)");
+  {
+    time_t t = time(nullptr);
+    char buf[64];
+    strftime(buf, sizeof buf, "%Y-%m-%d, %H-%M-%S", localtime(&t));
+    doc.add("Run Date/Time", std::string(buf));
+  }
+
+  // ---- driver.hpp:120-404 -------------------------------------------------
+  std::cout.width(30);
+  std::cout << "creating/filling mesh...";
+  std::cout.flush();
+  double t_start = mytimer(), t0 = mytimer();
+  simple_mesh_description<int> mesh(global_box, my_box);
+  double mesh_fill = mytimer() - t0;
+  double t_total = mytimer() - t_start;
+  std::cout << mesh_fill << "s, total time: " << t_total << std::endl;
+
+  MatrixType A;
+  double gen_structure = 0;
+  int gen_rc = 0;
+  RUN_TIMED_FUNCTION("generating matrix structure...",
+                     gen_rc = generate_matrix_structure(mesh, A), gen_structure, t_total);
+  if (gen_rc != 0) {
+    std::cout << "generate_matrix_structure failed (out of device memory)." << std::endl;
+    return 1;
+  }
+  const int local_nrows = (int)A.num_rows();
+  const int my_first_row = local_nrows > 0 ? 0 : -1;
+  VectorType b(my_first_row, local_nrows);
+  VectorType x(my_first_row, local_nrows);
+
+  double fe_assembly = 0;
+  RUN_TIMED_FUNCTION("assembling FE data...", assemble_FE_data(mesh, A, b, params), fe_assembly,
+                     t_total);
+  doc.add("Matrix structure generation");
+  doc.get("Matrix structure generation")->add("Mat-struc-gen Time", gen_structure);
+  doc.add("FE assembly");
+  doc.get("FE assembly")->add("FE assembly Time", fe_assembly);
+
+  double dirbc_time = 0;
+  RUN_TIMED_FUNCTION("imposing Dirichlet BC...",
+                     impose_dirichlet(0.0, A, b, params.nx + 1, params.ny + 1, params.nz + 1,
+                                      mesh.bc_rows_0),
+                     dirbc_time, t_total);
+  RUN_TIMED_FUNCTION("imposing Dirichlet BC...",
+                     impose_dirichlet(1.0, A, b, params.nx + 1, params.ny + 1, params.nz + 1,
+                                      mesh.bc_rows_1),
+                     dirbc_time, t_total);
+  double make_local_time = 0;
+  RUN_TIMED_FUNCTION("making matrix indices local...", make_local_matrix(A), make_local_time,
+                     t_total);
+
+  // compute_matrix_stats.hpp:80-108 (one rank)
+  const size_t global_nnz = A.num_nonzeros();
+  const size_t global_nrows_sz = A.num_rows();
+  {
+    Yaml* m = doc.add("Matrix attributes");
+    m->add("Global Nrows", global_nrows_sz);
+    m->add("Global NNZ", global_nnz);
+    const double invGB = 1.0 / (1024 * 1024 * 1024);
+    double memGB = invGB * global_nrows_sz * sizeof(int) + invGB * global_nrows_sz * sizeof(int) +
+                   invGB * global_nnz * sizeof(int) + invGB * global_nnz * sizeof(double);
+    m->add("Global Memory (GB)", memGB);
+    m->add("Pll Memory Overhead (MB)", 0);
+    m->add("Rows per proc MIN", global_nrows_sz);
+    m->add("Rows per proc MAX", global_nrows_sz);
+    m->add("Rows per proc AVG", (double)global_nrows_sz);
+    m->add("NNZ per proc MIN", global_nnz);
+    m->add("NNZ per proc MAX", global_nnz);
+    m->add("NNZ per proc AVG", (double)global_nnz);
+  }
+
+  int num_iters = 0;
+  double rnorm = 0;
+  double cg_times[NUM_TIMERS] = {0, 0, 0, 0, 0};
+  int verify_result = 0;
+  std::string title("CG solve");
+  if (!mixed) {
+    std::cout << "Starting CG solver ... " << std::endl;
+    cg_solve(A, b, x, matvec_std<MatrixType, VectorType>(), max_iters, tol, num_iters, rnorm,
+             cg_times);
+    std::cout << "Final Resid Norm: " << rnorm << std::endl;
+  } else {
+    std::cout << "Starting mixed-precision CG (FP32 inner, FP64 refinement) ... " << std::endl;
+    int outer = 0, inner = 0;
+    double rel = 0, t[5] = {0, 0, 0, 0, 0};
+    const int st = mfe_cg_solve_mixed(context(), A.h, b.h, x.h, 50, 20000, 1e-5, outer_tol,
+                                      &outer, &inner, &rel, t);
+    if (st != MFE_OK) std::cerr << "mixed solve: " << mfe_last_error(context()) << std::endl;
+    num_iters = inner;
+    rnorm = rel;
+    cg_times[TOTAL] = t[MFE_T_TOTAL];
+    title = "CG solve (mixed precision)";
+    doc.add("Mixed precision");
+    doc.get("Mixed precision")->add("Outer refinements", outer);
+    doc.get("Mixed precision")->add("Inner FP32 iterations", inner);
+    doc.get("Mixed precision")->add("Final relative residual", rel);
+    std::cout << "Final relative residual: " << rel << " after " << outer << " refinements, "
+              << inner << " FP32 iterations" << std::endl;
+  }
+  if (params.verify_solution > 0) {
+    std::cout << "verifying solution at ~ (0.5, 0.5, 0.5) ..." << std::endl;
+    double err = 0;
+    verify_result = verify_solution(mesh, x, 0.06, false, &err);
+    if (verify_result)
+      std::cout << "max absolute error is " << err << std::endl;
+    else
+      std::cout << "solution matches analytic solution to within 0.06 or better." << std::endl;
+  }
+
+  Yaml* g = doc.get("Global Run Parameters");
+  g->add("ScalarType", "double");
+  g->add("GlobalOrdinalType", "int");
+  g->add("LocalOrdinalType", "int");
+  Yaml* c = doc.add(title);
+  c->add("Iterations", num_iters);
+  c->add("Final Resid Norm", rnorm);
+  // driver.hpp:330-345, including its quirk of counting dot and waxpy flops
+  // over nx*ny*nz (elements) rather than rows.
+  const double global_nrows = (double)params.nx * params.ny * params.nz;
+  double mv_flops = global_nnz * 2.0 * (num_iters + 1);
+  double dot_flops = global_nrows * 2.0 * (2 * num_iters);
+  double waxpy_flops = global_nrows * 3.0 * (3 * num_iters + 2);
+  const double total_flops = mv_flops + dot_flops + waxpy_flops;
+  auto mflops = [](double f, double t) { return t > 1.e-4 ? 1.e-6 * (f / t) : -1.0; };
+  auto add_rate = [&](const char* k, double f, double t) {
+    const double m = mflops(f, t);
+    if (m >= 0) c->add(k, m);
+    else c->add(k, "inf");
+  };
+  c->add("WAXPY Time", cg_times[WAXPY]);
+  c->add("WAXPY Flops", waxpy_flops);
+  add_rate("WAXPY Mflops", waxpy_flops, cg_times[WAXPY]);
+  c->add("DOT Time", cg_times[DOT]);
+  c->add("DOT Flops", dot_flops);
+  add_rate("DOT Mflops", dot_flops, cg_times[DOT]);
+  c->add("MATVEC Time", cg_times[MATVEC]);
+  c->add("MATVEC Flops", mv_flops);
+  add_rate("MATVEC Mflops", mv_flops, cg_times[MATVEC]);
+  Yaml* tot = c->add("Total");
+  tot->add("Total CG Time", cg_times[TOTAL]);
+  tot->add("Total CG Flops", total_flops);
+  const double tm = mflops(total_flops, cg_times[TOTAL]);
+  if (tm >= 0) tot->add("Total CG Mflops", tm);
+  else tot->add("Total CG Mflops", "inf");
+  c->add("Time per iteration", num_iters > 0 ? cg_times[TOTAL] / num_iters : 0.0);
+  doc.add("Total Program Time", mytimer() - start_time);
+
+  // YAML_Doc.cpp:32-72 file naming
+  std::ostringstream fname;
+  fname << "miniFE." << params.nx << "x" << params.ny << "x" << params.nz << ".";
+  if (!params.name.empty()) fname << params.name << ".";
+  {
+    time_t t = time(nullptr);
+    char sdate[32];
+    strftime(sdate, sizeof sdate, "%Y:%m:%d-%H:%M:%S", localtime(&t));
+    fname << sdate << ".yaml";
+  }
+  std::string yaml = "Mini-Application Name: miniFE\nMini-Application Version: 2.0\n";
+  for (auto& k : doc.kids) yaml += k->print("");
+  std::ofstream(fname.str()) << yaml;
+  return verify_result;
+}
