@@ -21,8 +21,10 @@ def run_driver(tmp_path, *args):
     doc = {}
     stack = []
     for line in open(os.path.join(tmp_path, yamls[0])):
+        line = line.rstrip("\n")
         depth = (len(line) - len(line.lstrip(" "))) // 2
-        key, _, val = line.strip().partition(": ")
+        key, _, val = line.strip(" ").partition(": ")
+        key = key.rstrip(":")
         stack = stack[:depth] + [key]
         doc["/".join(stack)] = val
     return r, doc
