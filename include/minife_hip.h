@@ -86,6 +86,10 @@ int mfe_csr_download(mfe_ctx* ctx, const mfe_csr* A, int* rows, int* row_offsets
 int mfe_vec_create(mfe_ctx* ctx, int64_t n, mfe_vec** out);
 void mfe_vec_destroy(mfe_vec* v);
 int64_t mfe_vec_size(const mfe_vec* v);
+/* A vector over caller-owned device memory (e.g. a torch tensor's storage);
+ * mfe_vec_destroy releases the handle only.  The memory must stay valid and
+ * must not be written by other streams while the library uses it. */
+int mfe_vec_wrap(mfe_ctx* ctx, double* device_ptr, int64_t n, mfe_vec** out);
 int mfe_vec_upload(mfe_ctx* ctx, mfe_vec* v, const double* host, int64_t n);
 int mfe_vec_download(mfe_ctx* ctx, const mfe_vec* v, double* host, int64_t n);
 int mfe_vec_fill(mfe_ctx* ctx, mfe_vec* v, double value);
@@ -156,6 +160,19 @@ int mfe_waxpby(mfe_ctx* ctx, double alpha, const mfe_vec* x, double beta, const 
 int mfe_cg_solve(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x, int max_iter,
                  double tolerance, unsigned flags, int* num_iters, double* normr,
                  double* times, double* history);
+
+/* ---- adjoint (extension; no counterpart in the reference) ------------------
+ * grad[e] = -lambda_e^T K_e(1) utilde_e for every element e of the global box:
+ * K_e(1) is Hex8::diffusionMatrix_symm with k = 1 (Hex8.hpp:259-361), the
+ * derivative of the assembled operator with respect to the element
+ * conductivity kappa_e; lambda and utilde are nodal vectors (n = nodes), grad
+ * has one entry per element.  With lambda the adjoint solution (zero on
+ * Dirichlet rows) and utilde the solution with its bc_rows_0 entries set to 0
+ * (the values the reference's Dirichlet lifting uses), grad is dL/dkappa.
+ * Sum order per element: for i: t = sum_j K_ij utilde_j (j ascending);
+ * s += lambda_i t; grad = -s. */
+int mfe_element_gradient(mfe_ctx* ctx, const int global_box[6], const mfe_vec* lambda,
+                         const mfe_vec* utilde, mfe_vec* grad);
 
 /* ---- mixed precision (extension; no counterpart in the reference) --------
  * Iterative refinement around cg_solve: the inner solve is unpreconditioned

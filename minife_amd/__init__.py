@@ -41,7 +41,7 @@ EXPORTS = [
     "mfe_device_info", "mfe_timer_start", "mfe_timer_stop",
     "mfe_csr_create", "mfe_csr_destroy", "mfe_csr_shape", "mfe_csr_upload", "mfe_csr_download",
     "mfe_vec_create", "mfe_vec_destroy", "mfe_vec_size", "mfe_vec_upload", "mfe_vec_download",
-    "mfe_vec_fill",
+    "mfe_vec_fill", "mfe_vec_wrap", "mfe_element_gradient",
     "mfe_generate_matrix_structure", "mfe_perform_element_loop", "mfe_assemble_FE_data",
     "mfe_impose_dirichlet", "mfe_make_local_matrix",
     "mfe_matvec", "mfe_matvec_and_dot", "mfe_dot", "mfe_dot_replay", "mfe_waxpby",
@@ -91,6 +91,8 @@ def lib() -> ctypes.CDLL:
         "mfe_vec_upload": ([_vp, _vp, _vp, _i64], ctypes.c_int),
         "mfe_vec_download": ([_vp, _vp, _vp, _i64], ctypes.c_int),
         "mfe_vec_fill": ([_vp, _vp, ctypes.c_double], ctypes.c_int),
+        "mfe_vec_wrap": ([_vp, _vp, _i64, ctypes.POINTER(_vp)], ctypes.c_int),
+        "mfe_element_gradient": ([_vp, _int6, _vp, _vp, _vp], ctypes.c_int),
         "mfe_generate_matrix_structure": ([_vp, _int6, _int6, _vp], ctypes.c_int),
         "mfe_perform_element_loop": ([_vp, _int6, _int6, _vp, _vp, _vp], ctypes.c_int),
         "mfe_assemble_FE_data": ([_vp, _int6, _int6, _vp, _vp, _vp], ctypes.c_int),
@@ -254,6 +256,20 @@ class Vector:
         self.ctx.check(lib().mfe_vec_create(self.ctx.h, int(n), ctypes.byref(h)), "vec_create")
         self.h = h
         self.n = int(n)
+
+    @classmethod
+    def wrap(cls, device_ptr: int, n: int, ctx: Context | None = None, owner=None):
+        """A view over caller-owned device memory (e.g. a torch tensor's
+        data_ptr()); `owner` is kept alive with the view."""
+        v = cls.__new__(cls)
+        v.ctx = ctx or default_context()
+        h = _vp()
+        v.ctx.check(lib().mfe_vec_wrap(v.ctx.h, _vp(int(device_ptr)), int(n), ctypes.byref(h)),
+                    "vec_wrap")
+        v.h = h
+        v.n = int(n)
+        v._owner = owner
+        return v
 
     @classmethod
     def from_numpy(cls, a, ctx: Context | None = None):
@@ -468,6 +484,13 @@ def time_matvec(A: CSRMatrix, x: Vector, y: Vector, variant: int = 1, reps: int 
     A.ctx.check(lib().mfe_time_matvec(A.ctx.h, A.h, x.h, y.h, int(variant), int(reps),
                                       ctypes.byref(ms)), "time_matvec")
     return ms.value
+
+
+def element_gradient(mesh: simple_mesh_description, lam: Vector, utilde: Vector,
+                     grad: Vector) -> None:
+    """grad[e] = -lambda_e^T K_e(1) utilde_e (the adjoint element gradient)."""
+    lam.ctx.check(lib().mfe_element_gradient(lam.ctx.h, _box6(mesh.global_box), lam.h, utilde.h,
+                                             grad.h), "element_gradient")
 
 
 def soln(x: float, y: float, z: float, max_p: int = 300, max_q: int = 300) -> float:

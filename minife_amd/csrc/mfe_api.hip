@@ -286,8 +286,23 @@ void mfe_vec_destroy(mfe_vec* v) {
   if (!v) return;
   hipSetDevice(v->ctx->device);
   hipStreamSynchronize(v->ctx->stream);
-  hipFree(v->d);
+  if (v->owns) hipFree(v->d);
   delete v;
+}
+
+int mfe_vec_wrap(mfe_ctx* ctx, double* device_ptr, int64_t n, mfe_vec** out) {
+  if (!ctx || !out || n < 0 || (n > 0 && !device_ptr)) return MFE_BAD_ARG;
+  hipPointerAttribute_t attr;
+  if (n > 0 && (hipPointerGetAttributes(&attr, device_ptr) != hipSuccess ||
+                attr.type != hipMemoryTypeDevice || attr.device != ctx->device))
+    return fail(ctx, MFE_BAD_ARG, "mfe_vec_wrap: not device memory of this context's GPU");
+  mfe_vec* v = new mfe_vec();
+  v->ctx = ctx;
+  v->n = n;
+  v->d = device_ptr;
+  v->owns = false;
+  *out = v;
+  return MFE_OK;
 }
 
 int64_t mfe_vec_size(const mfe_vec* v) { return v ? v->n : -1; }

@@ -323,6 +323,44 @@ __global__ __launch_bounds__(kBlock) void dirichlet_kernel(int64_t n, const int*
   b[row] -= sum * value;
 }
 
+// grad[e] = -lambda_e^T K_e(1) utilde_e, one thread per element.
+__global__ __launch_bounds__(kBlock) void element_gradient_kernel(
+    int Nx, int Ny, int Nz, double hx, double hy, double hz, HexTables tab,
+    const double* __restrict__ lambda, const double* __restrict__ ut, double* __restrict__ grad,
+    int64_t ne) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int ex = (int)(e % Nx);
+  const int ey = (int)((e / Nx) % Ny);
+  const int ez = (int)(e / ((int64_t)Nx * Ny));
+  const double x0 = 1.0 * ex / Nx;
+  const double y0 = 1.0 * ey / Ny;
+  const double z0 = (1.0 * ez) / Nz;
+  const double x1 = x0 + hx, y1 = y0 + hy, z1 = z0 + hz;
+  const double c[24] = {x0, y0, z0, x1, y0, z0, x1, y1, z0, x0, y1, z0,
+                        x0, y0, z1, x1, y0, z1, x1, y1, z1, x0, y1, z1};
+  double K[36], F[8];
+  hex8_element(c, tab, 1.0, K, F);
+  const int64_t gx = Nx + 1, gxy = (int64_t)(Nx + 1) * (Ny + 1);
+  const int64_t node0 = ex + gx * ey + gxy * ez;
+  double lam[8], u[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int64_t id = node0 + corner_x(i) + gx * corner_y(i) + gxy * corner_z(i);
+    lam[i] = lambda[id];
+    u[i] = ut[id];
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) t += K[packed(i, j)] * u[j];
+    s += lam[i] * t;
+  }
+  grad[e] = -s;
+}
+
 static bool box_ok(const int* g) {
   return g && g[0] == 0 && g[2] == 0 && g[4] == 0 && g[1] > 0 && g[3] > 0 && g[5] > 0;
 }
@@ -454,3 +492,21 @@ int mfe_impose_dirichlet(mfe_ctx* ctx, double value, mfe_csr* A, mfe_vec* b, int
 }
 
 }  // extern "C"
+
+extern "C" int mfe_element_gradient(mfe_ctx* ctx, const int gbox[6], const mfe_vec* lambda,
+                                    const mfe_vec* utilde, mfe_vec* grad) {
+  if (!ctx || !box_ok(gbox) || !lambda || !utilde || !grad) return MFE_BAD_ARG;
+  const int Nx = gbox[1], Ny = gbox[3], Nz = gbox[5];
+  const int64_t n = (int64_t)(Nx + 1) * (Ny + 1) * (Nz + 1);
+  const int64_t ne = (int64_t)Nx * Ny * Nz;
+  if (lambda->n != n || utilde->n != n || grad->n != ne)
+    return fail(ctx, MFE_BAD_ARG, "element_gradient: lambda/utilde need one value per node, "
+                                  "grad one per element");
+  MFE_CHECK(ctx, hipSetDevice(ctx->device));
+  HexTables tab;
+  host_tables(tab);
+  element_gradient_kernel<<<(unsigned)cdiv(ne, kBlock), kBlock, 0, ctx->stream>>>(
+      Nx, Ny, Nz, 1.0 / Nx, 1.0 / Ny, 1.0 / Nz, tab, lambda->d, utilde->d, grad->d, ne);
+  MFE_LAUNCHED(ctx, "element_gradient_kernel");
+  return MFE_OK;
+}

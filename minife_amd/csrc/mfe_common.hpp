@@ -57,6 +57,7 @@ struct mfe_vec {
   mfe_ctx* ctx = nullptr;
   int64_t n = 0;
   double* d = nullptr;
+  bool owns = true;  // false for mfe_vec_wrap views
 };
 
 namespace mfe {

@@ -66,6 +66,7 @@ def lib():
     L.orc_cg_solve.argtypes = [_i64, _i32p, _i32p, _f64p, _f64p, _f64p, ctypes.c_int,
                                ctypes.c_double, ctypes.POINTER(ctypes.c_int),
                                ctypes.POINTER(ctypes.c_double), vp, vp]
+    L.orc_element_gradient.argtypes = [_i32p, _f64p, _f64p, _f64p]
     L.orc_soln.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, ctypes.c_int]
     L.orc_soln.restype = ctypes.c_double
     L.orc_verify_solution.argtypes = [_i32p, _f64p, ctypes.c_double, ctypes.c_int,
@@ -221,6 +222,38 @@ def verify_solution(nx, ny, nz, x, tol=0.06, whole=False):
     rc = lib().orc_verify_solution(box(nx, ny, nz), np.ascontiguousarray(x), float(tol),
                                    int(whole), ctypes.byref(e))
     return rc, e.value
+
+
+def element_gradient(nx, ny, nz, lam, utilde) -> np.ndarray:
+    g = np.empty(nx * ny * nz, np.float64)
+    lib().orc_element_gradient(box(nx, ny, nz), np.ascontiguousarray(lam, dtype=np.float64),
+                               np.ascontiguousarray(utilde, dtype=np.float64), g)
+    return g
+
+
+def solve_kappa(nx, ny, nz, kappa, rel_tol=1e-13, max_iter=20000):
+    """FP64 CG solution of the kappa-weighted problem (absolute tol = rel_tol*||b||)."""
+    P = build_problem(nx, ny, nz, kappa)
+    tol = rel_tol * float(np.sqrt(dot(P["b"], P["b"])))
+    r = cg_solve(P["A"], P["b"], max_iter=max_iter, tol=tol)
+    return P, r["x"]
+
+
+def loss_and_gradient(nx, ny, nz, kappa, u_target, rel_tol=1e-13):
+    """L = 0.5 ||u(kappa) - u_target||^2 and dL/dkappa by the adjoint method:
+    A lambda = (u - u_target) with Dirichlet rows zeroed; grad = -lambda_e^T K_e(1) ut_e,
+    ut = u with bc_rows_0 entries zeroed (the values the Dirichlet lifting uses)."""
+    P, u = solve_kappa(nx, ny, nz, kappa, rel_tol)
+    d = u - u_target
+    L = 0.5 * float(np.dot(d, d))
+    rhs = d.copy()
+    bc = np.union1d(bc_rows(nx, ny, nz, 0), bc_rows(nx, ny, nz, 1))
+    rhs[bc] = 0.0
+    tol = rel_tol * float(np.sqrt(dot(rhs, rhs)))
+    lam = cg_solve(P["A"], rhs, max_iter=20000, tol=tol)["x"]
+    ut = u.copy()
+    ut[bc_rows(nx, ny, nz, 0)] = 0.0
+    return L, element_gradient(nx, ny, nz, lam, ut), u
 
 
 def build_problem(nx, ny, nz, kappa=None):

@@ -491,6 +491,30 @@ int orc_cg_solve(int64_t n, const int* ro, const int* cols, const double* vals,
   return status;
 }
 
+static int packed_idx(int a, int b) {
+  if (a > b) { int t = a; a = b; b = t; }
+  return a * 8 - a * (a - 1) / 2 + (b - a);
+}
+
+void orc_element_gradient(const int gbox[6], const double* lambda, const double* ut,
+                          double* grad) {
+  double gv[192], coords[24], K[36];
+  int ids[8];
+  orc_compute_gradient_values(gv);
+  const int ne = gbox[1] * gbox[3] * gbox[5];
+  for (int e = 0; e < ne; ++e) {
+    orc_get_elem_nodes_and_coords(gbox, e, ids, coords);
+    orc_diffusion_matrix_symm(coords, gv, 1.0, K);
+    double s = 0.0;
+    for (int i = 0; i < 8; ++i) {
+      double t = 0.0;
+      for (int j = 0; j < 8; ++j) t += K[packed_idx(i, j)] * ut[ids[j]];
+      s += lambda[ids[i]] * t;
+    }
+    grad[e] = -s;
+  }
+}
+
 /* analytic_soln.hpp:51-112 */
 static const double PI = 3.141592653589793238462;
 double orc_soln(double x, double y, double z, int max_p, int max_q) {

@@ -105,6 +105,12 @@ int orc_cg_solve(int64_t nrows, const int* row_offsets, const int* cols,
                  double tolerance, int* num_iters, double* normr, double* history,
                  double* times);
 
+/* Adjoint element gradient (no reference counterpart): grad[e] =
+ * -lambda_e^T K_e(1) utilde_e with K_e(1) = diffusionMatrix_symm at k = 1;
+ * per element: for i: t = sum_j K_ij utilde_j; s += lambda_i t; grad = -s.
+ * Pinned by central finite differences of the oracle's own solve. */
+void orc_element_gradient(const int gbox[6], const double* lambda, const double* utilde,
+                          double* grad);
 /* ref/fem/analytic_soln.hpp:80-112 */
 double orc_soln(double x, double y, double z, int max_p, int max_q);
 /* ref/fem/verify_solution.hpp:55-174; returns rc, *max_err the max abs error. */

@@ -346,3 +346,55 @@ def test_analytic_solution(oracle_mod):
                + s(1 - z, x, y))
         assert abs(tot - 1.0) < 1e-10
         assert abs(s(x, y, z) - s(x, 1 - y, z)) < 1e-12  # mirror symmetry in y
+
+
+def _kappa_field(n, seed, sigma=0.5):
+    return np.exp(np.random.default_rng(seed).normal(0.0, sigma, n))
+
+
+def _fd_grad(oracle_mod, dims, kappa, ut, e, h=1e-6):
+    """Central difference of L = 0.5||u - ut||^2 in kappa[e]; the difference
+    of squares is formed as (up - um).(up + um - 2 ut) to avoid cancellation."""
+    kp, km = kappa.copy(), kappa.copy()
+    kp[e] += h
+    km[e] -= h
+    up = oracle_mod.solve_kappa(*dims, kp)[1]
+    um = oracle_mod.solve_kappa(*dims, km)[1]
+    return 0.5 * float(np.dot(up - um, up + um - 2 * ut)) / (2 * h)
+
+
+def test_adjoint_gradient_finite_differences(oracle_mod):
+    """The adjoint gradient equals central differences of the oracle's own
+    solve on every element of a ragged mesh (covers elements on all faces,
+    including the x=1 face edges that are in both Dirichlet sets)."""
+    dims = (3, 4, 5)
+    ne = 60
+    kappa = _kappa_field(ne, 2)
+    ut = oracle_mod.solve_kappa(*dims, _kappa_field(ne, 3))[1]
+    L, g, u = oracle_mod.loss_and_gradient(*dims, kappa, ut)
+    assert L > 0
+    scale = np.max(np.abs(g))
+    for e in range(ne):
+        fd = _fd_grad(oracle_mod, dims, kappa, ut, e)
+        assert abs(fd - g[e]) <= 1e-6 * scale + 1e-5 * abs(g[e]), (e, fd, g[e])
+
+
+def test_adjoint_uniform_scaling_identity(oracle_mod):
+    """Scaling every kappa by c scales K but not the source or the identity
+    Dirichlet rows, so dL/dc at c=1 = sum_e kappa_e dL/dkappa_e = -lambda_F.f_F
+    with f the pre-BC source vector and lambda the adjoint solution (zero on
+    Dirichlet rows).  A gradient that kept the doubly-constrained edge nodes at
+    1 or dropped a lifting term breaks this."""
+    dims = (6, 5, 7)
+    ne = 6 * 5 * 7
+    kappa = _kappa_field(ne, 2)
+    ut = oracle_mod.solve_kappa(*dims, _kappa_field(ne, 3))[1]
+    L, g, u = oracle_mod.loss_and_gradient(*dims, kappa, ut)
+    P = oracle_mod.build_problem(*dims, kappa)
+    rhs = u - ut
+    bc = np.union1d(oracle_mod.bc_rows(*dims, 0), oracle_mod.bc_rows(*dims, 1))
+    rhs[bc] = 0.0
+    lam = np.linalg.solve(_dense(P["A"]), rhs)
+    lhs = float(np.dot(kappa, g))
+    rhs_val = -float(np.dot(lam, P["b_prebc"]))
+    assert abs(lhs - rhs_val) <= 1e-9 * abs(rhs_val), (lhs, rhs_val)
