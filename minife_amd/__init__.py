@@ -63,6 +63,31 @@ _i64 = ctypes.c_int64
 _int6 = ctypes.c_int * 6
 
 
+def _share_torch_hip_runtime() -> None:
+    """Keep one HIP runtime per process.  PyTorch wheels bundle their own
+    libamdhip64/libhsa-runtime64 and load them by file name, so a process that
+    loaded the system runtime first ends up with two HSA runtimes and torch
+    then finds no GPU.  Loading torch's copy first (without importing torch)
+    makes this library bind to it through the shared soname, so tensors and
+    library vectors live in one runtime whatever the import order.
+    MINIFE_SYSTEM_HIP_RUNTIME=1 keeps the system runtime (processes that
+    never use torch)."""
+    if os.environ.get("MINIFE_SYSTEM_HIP_RUNTIME") == "1":
+        return
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return
+    if spec is None or not spec.submodule_search_locations:
+        return
+    for loc in spec.submodule_search_locations:
+        path = os.path.join(loc, "lib", "libamdhip64.so")
+        if os.path.exists(path):
+            ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+            return
+
+
 def lib() -> ctypes.CDLL:
     """Load libminife_hip.so (raises if it has not been built)."""
     global _lib
@@ -70,6 +95,7 @@ def lib() -> ctypes.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m minife_amd.build`")
+    _share_torch_hip_runtime()
     L = ctypes.CDLL(LIB_PATH)
     sig = {
         "mfe_ctx_create": ([ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),

@@ -61,3 +61,19 @@ def test_mesh_rejects_sub_boxes():
         simple_mesh_description([0, 4, 0, 4, 0, 4], [0, 2, 0, 4, 0, 4])
     with pytest.raises(ValueError):
         simple_mesh_description([1, 4, 0, 4, 0, 4])
+
+
+def test_one_hip_runtime_per_process():
+    """Loading the library and then torch (or the other way round) maps a
+    single libamdhip64: tensors and library vectors share one runtime."""
+    import subprocess
+    import sys
+    code = ("import minife_amd; minife_amd.lib(); import torch\n"
+            "m = open('/proc/self/maps').read().splitlines()\n"
+            "print(len({l.split()[-1] for l in m if 'libamdhip64' in l}))\n")
+    for src in (code, code.replace("import minife_amd; minife_amd.lib(); import torch",
+                                   "import torch; import minife_amd; minife_amd.lib()")):
+        out = subprocess.run([sys.executable, "-c", src], cwd=ROOT, capture_output=True,
+                             text=True, timeout=300)
+        assert out.returncode == 0, out.stderr
+        assert out.stdout.strip() == "1"
