@@ -148,7 +148,7 @@ def test_torch_autograd(mfe, adj, oracle_mod):
     loss = 0.5 * torch.sum((u - ut_t) ** 2)
     loss.backward()
     g = k_t.grad.detach().cpu().numpy()
-    assert abs(float(loss) - L0) <= 1e-8 * L0
+    assert abs(loss.item() - L0) <= 1e-8 * L0
     assert np.max(np.abs(g - g0)) <= 1e-8 * np.max(np.abs(g0))
     # a linear functional w.u: its gradient is the oracle's gradient of
     # 0.5||u - (u0 - w)||^2 at the same kappa (residual d = w)
