@@ -40,15 +40,27 @@ def spmv_bytes(n: int, nnz: int, value_bytes: int = 8) -> int:
     return (value_bytes + 4) * nnz + 4 * (n + 1) + 2 * value_bytes * n
 
 
-def measured_traffic(N: int):
+def sell_bytes(n: int, plan: dict, value_bytes: int = 8) -> int:
+    """HBM bytes of one SELL-64 SpMV: every stored entry (padding included)
+    as value + int32 column, one length byte per row slot, the slice offsets,
+    x read once and y written once."""
+    slots, ns = plan["stored"], plan["slices"]
+    return (value_bytes + 4) * slots + 64 * ns + 4 * (ns + 1) + 2 * value_bytes * n
+
+
+def measured_traffic(N: int, kernel: str):
     """HBM bytes per launch of the CG SpMV kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary for this workload (calibrated against a
-    kernel of known size; see the summary for the commands), else None."""
+    kernel of known size; see the summary for the commands), else None --
+    also when the summary was taken of another kernel."""
     path = os.path.join(ROOT, "profiles", f"spmv_cg_{N}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("traffic_bytes_per_launch")
+        d = json.load(f)
+    if not d.get("kernel", "").startswith(kernel):
+        return None
+    return d.get("traffic_bytes_per_launch")
 
 
 def cpu_baseline(A_dev, b_dev, n, nnz, iters: int):
@@ -119,6 +131,10 @@ def main():
     t_bc = time.perf_counter() - t0
     mfe.make_local_matrix(A)
     log(f"assembled n={n} nnz={nnz}: struct {t_struct:.3f}s, FE {t_asm:.3f}s, BC {t_bc:.3f}s")
+    t0 = time.perf_counter()
+    plan = mfe.spmv_plan(A)
+    t_plan = time.perf_counter() - t0
+    log(f"spmv plan {plan} built in {t_plan:.3f}s")
 
     # ---- CG steps ----------------------------------------------------------
     x = mfe.Vector(n, ctx)
@@ -146,12 +162,20 @@ def main():
     ms_cg = mfe.time_matvec(A, xs, ys, variant=1, reps=args.spmv_reps)
     ms_std = mfe.time_matvec(A, xs, ys, variant=0, reps=args.spmv_reps)
     ms_32 = mfe.time_matvec(A, xs, ys, variant=2, reps=args.spmv_reps)
-    byt = spmv_bytes(n, nnz)
+    ms_csr = mfe.time_matvec(A, xs, ys, variant=3, reps=args.spmv_reps)
+    ms_csr32 = mfe.time_matvec(A, xs, ys, variant=4, reps=args.spmv_reps)
+    sell = plan["kernel"] == "sell64"
+    byt = sell_bytes(n, plan) if sell else spmv_bytes(n, nnz)
+    byt32 = sell_bytes(n, plan, 4) if sell else spmv_bytes(n, nnz, 4)
     gbs_cg = byt / (ms_cg * 1e-3) / 1e9
     gbs_std = byt / (ms_std * 1e-3) / 1e9
-    gbs_32 = spmv_bytes(n, nnz, 4) / (ms_32 * 1e-3) / 1e9
-    log(f"spmv (cg kernel) {ms_cg:.4f} ms -> {gbs_cg:.1f} GB/s; matvec_std {ms_std:.4f} ms "
-        f"-> {gbs_std:.1f} GB/s; fp32 {ms_32:.4f} ms -> {gbs_32:.1f} GB/s")
+    gbs_32 = byt32 / (ms_32 * 1e-3) / 1e9
+    gbs_csr = spmv_bytes(n, nnz) / (ms_csr * 1e-3) / 1e9
+    gbs_csr32 = spmv_bytes(n, nnz, 4) / (ms_csr32 * 1e-3) / 1e9
+    log(f"spmv ({plan['kernel']}, cg kernel) {ms_cg:.4f} ms -> {gbs_cg:.1f} GB/s; matvec_std "
+        f"{ms_std:.4f} ms -> {gbs_std:.1f} GB/s; fp32 {ms_32:.4f} ms -> {gbs_32:.1f} GB/s; "
+        f"csr kernels: fp64 {ms_csr:.4f} ms ({gbs_csr:.1f} GB/s), fp32 {ms_csr32:.4f} ms "
+        f"({gbs_csr32:.1f} GB/s)")
 
     # ---- time to ||r||/||b|| <= 1e-10 ----------------------------------------
     tol_run = None
@@ -215,12 +239,18 @@ def main():
                    "nx": N, "ny": N, "nz": N, "nrows": n, "nnz": nnz,
                    "cg_iters_per_step": args.iters},
         "roofline": {"bound": "hbm", "achieved": gbs_cg, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs_cg / HBM_PEAK_GBS, "traffic": measured_traffic(N),
-                     "kernel": "spmv_stream_kernel<false,true> (CG SpMV + p.Ap partials)",
+                     "frac": gbs_cg / HBM_PEAK_GBS, "traffic": measured_traffic(
+                         N, "spmv_sell_kernel<double" if sell else "spmv_stream_kernel<double"),
+                     "kernel": ("spmv_sell_kernel<double,false,kDot>" if sell else
+                                "spmv_stream_kernel<double,false,kDot>")
+                               + " (CG SpMV + p.Ap partials)",
                      "bytes_per_launch": byt, "avg_ms": ms_cg},
         "spmv_GBs": gbs_cg,
         "spmv_std_GBs": gbs_std,
         "spmv_fp32_GBs": gbs_32,
+        "spmv_plan": dict(plan, build_s=t_plan),
+        "spmv_csr_kernels": {"fp64_ms": ms_csr, "fp64_GBs": gbs_csr, "fp32_ms": ms_csr32,
+                             "fp32_GBs": gbs_csr32},
         "assembly": {"struct_s": t_struct, "fe_s": t_asm, "dirichlet_s": t_bc,
                      "elements_per_s": ne / t_asm},
         "time_to_1e-10": tol_run,

@@ -128,6 +128,21 @@ int mfe_impose_dirichlet(mfe_ctx* ctx, double prescribed_value, mfe_csr* A, mfe_
 int mfe_make_local_matrix(mfe_ctx* ctx, mfe_csr* A);
 
 /* ---- solve --------------------------------------------------------------- */
+/* SpMV storage plan (no reference counterpart: the reference always runs its
+ * CSR loop).  MFE_SPMV_AUTO (default) keeps a SELL-64 copy of A beside the CSR
+ * arrays -- slices of 64 rows, one row per lane, slot j of a slice stored as
+ * 64 consecutive values/columns -- built on first use and rebuilt after the
+ * values or structure change; matrices with a row longer than 27 entries, or
+ * whose copy does not fit in device memory, stay on the CSR kernels.
+ * MFE_SPMV_CSR frees the copy and uses the CSR kernels only.  Every format
+ * sums each row in stored column order, so results are bit-identical. */
+#define MFE_SPMV_AUTO 0
+#define MFE_SPMV_CSR 1
+/* Sets A's format and builds its FP64 plan now (so later solves do not pay
+ * for it).  info (nullable) receives {kernel in use: 0 CSR row-stream,
+ * 1 SELL-64, 2 CSR row-per-thread; stored entries incl. padding; slices;
+ * longest row}. */
+int mfe_spmv_plan(mfe_ctx* ctx, mfe_csr* A, int format, int64_t info[4]);
 /* matvec_std<CSRMatrix>::operator()(A, x, y), SparseMatrix_functions.hpp:545-578.
  * Each row's products are summed in stored column order: bit-identical. */
 int mfe_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y);
@@ -197,7 +212,8 @@ int mfe_matvec_fp32(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y
  * kernel (row sums start at 0*y[row]); variant 1 = the kernel inside the fast
  * cg_solve (sums start at +0.0, plus workgroup partials of x.y); variant 2 =
  * the FP32 kernel inside mfe_cg_solve_mixed's inner CG (x is used as its
- * FP32 copy, converted before the timed launches). */
+ * FP32 copy, converted before the timed launches).  Variants 3 and 4 are 1
+ * and 2 on the CSR kernels whatever A's plan (for comparison). */
 int mfe_time_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y, int variant,
                     int reps, double* avg_ms);
 

@@ -33,6 +33,8 @@ LIB_PATH = os.path.join(_HERE, "libminife_hip.so")
 MFE_OK, MFE_BREAKDOWN, MFE_NOT_LOCAL, MFE_OOM, MFE_HIP_ERROR, MFE_BAD_ARG, MFE_UNSUPPORTED = range(7)
 STATUS_NAMES = ["OK", "BREAKDOWN", "NOT_LOCAL", "OOM", "HIP_ERROR", "BAD_ARG", "UNSUPPORTED"]
 CG_REPLAY, CG_TOL_RELATIVE, CG_EAGER = 0x1, 0x2, 0x4
+SPMV_AUTO, SPMV_CSR = 0, 1
+SPMV_KERNELS = ["csr-stream", "sell64", "csr-rowwise"]
 T_WAXPY, T_DOT, T_MATVEC, T_MATVECDOT, T_TOTAL = range(5)
 
 # Every extern "C" symbol declared in include/minife_hip.h.
@@ -43,7 +45,7 @@ EXPORTS = [
     "mfe_vec_create", "mfe_vec_destroy", "mfe_vec_size", "mfe_vec_upload", "mfe_vec_download",
     "mfe_vec_fill", "mfe_vec_wrap", "mfe_element_gradient",
     "mfe_generate_matrix_structure", "mfe_perform_element_loop", "mfe_assemble_FE_data",
-    "mfe_impose_dirichlet", "mfe_make_local_matrix",
+    "mfe_impose_dirichlet", "mfe_make_local_matrix", "mfe_spmv_plan",
     "mfe_matvec", "mfe_matvec_and_dot", "mfe_dot", "mfe_dot_replay", "mfe_waxpby",
     "mfe_cg_solve", "mfe_cg_solve_mixed", "mfe_matvec_fp32", "mfe_time_matvec", "mfe_soln",
     "mfe_verify_solution",
@@ -125,6 +127,7 @@ def lib() -> ctypes.CDLL:
         "mfe_impose_dirichlet": ([_vp, ctypes.c_double, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, _vp, _i64], ctypes.c_int),
         "mfe_make_local_matrix": ([_vp, _vp], ctypes.c_int),
+        "mfe_spmv_plan": ([_vp, _vp, ctypes.c_int, _vp], ctypes.c_int),
         "mfe_matvec": ([_vp, _vp, _vp, _vp], ctypes.c_int),
         "mfe_matvec_and_dot": ([_vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "mfe_dot": ([_vp, _vp, _vp, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
@@ -414,6 +417,15 @@ def make_local_matrix(A: CSRMatrix) -> None:
     A.ctx.check(lib().mfe_make_local_matrix(A.ctx.h, A.h), "make_local_matrix")
 
 
+def spmv_plan(A: CSRMatrix, fmt: int = SPMV_AUTO) -> dict:
+    """Choose A's SpMV storage (SPMV_AUTO: SELL-64 copy when every row has at
+    most 27 entries; SPMV_CSR: CSR kernels only) and build it now."""
+    info = (ctypes.c_int64 * 4)()
+    A.ctx.check(lib().mfe_spmv_plan(A.ctx.h, A.h, int(fmt), info), "spmv_plan")
+    return dict(kernel=SPMV_KERNELS[info[0]], stored=int(info[1]), slices=int(info[2]),
+                longest_row=int(info[3]))
+
+
 def matvec(A: CSRMatrix, x: Vector, y: Vector) -> None:
     A.ctx.check(lib().mfe_matvec(A.ctx.h, A.h, x.h, y.h), "matvec")
 
@@ -505,7 +517,8 @@ def matvec_fp32(A: CSRMatrix, x: Vector, y: Vector) -> None:
 def time_matvec(A: CSRMatrix, x: Vector, y: Vector, variant: int = 1, reps: int = 20) -> float:
     """Mean SpMV kernel time in ms over `reps` back-to-back launches (HIP events
     on the context's stream); variant 1 is the kernel inside the fast cg_solve,
-    variant 2 the FP32 kernel inside cg_solve_mixed."""
+    variant 2 the FP32 kernel inside cg_solve_mixed, 3 and 4 the same two on
+    the CSR kernels whatever A's plan."""
     ms = ctypes.c_double(0)
     A.ctx.check(lib().mfe_time_matvec(A.ctx.h, A.h, x.h, y.h, int(variant), int(reps),
                                       ctypes.byref(ms)), "time_matvec")

@@ -143,9 +143,9 @@ static void csr_release(mfe_csr* A) {
   A->rows = A->row_offsets = A->cols = nullptr;
   A->vals = nullptr;
   A->vals32 = nullptr;
-  A->vals32_valid = false;
+  sell_release(A);
+  csr_structure_changed(A);
   A->nrows = A->nnz = 0;
-  A->max_block_nnz = -1;
   A->has_local_indices = false;
   A->num_cols = 0;
 }

@@ -393,7 +393,7 @@ int mfe_generate_matrix_structure(mfe_ctx* ctx, const int gbox[6], const int lbo
   MFE_CHECK(ctx, hipMemsetAsync(A->vals, 0, sizeof(double) * (size_t)nnz, ctx->stream));
   A->has_local_indices = false;
   A->num_cols = 0;
-  A->max_block_nnz = -1;
+  csr_structure_changed(A);
   return plan_spmv(ctx, A);
 }
 
@@ -430,7 +430,7 @@ int mfe_perform_element_loop(mfe_ctx* ctx, const int gbox[6], const int ebox[6],
                                                                           ne, A->vals, b->d);
     le = hipGetLastError();
   }
-  A->vals32_valid = false;
+  csr_values_changed(A);
   if (le != hipSuccess) st = hip_fail(ctx, le, "element/gather kernels");
   hipError_t se = hipStreamSynchronize(ctx->stream);
   if (st == MFE_OK && se != hipSuccess) st = hip_fail(ctx, se, "assembly");
@@ -479,7 +479,7 @@ int mfe_impose_dirichlet(mfe_ctx* ctx, double value, mfe_csr* A, mfe_vec* b, int
       bc_flag_kernel<<<(unsigned)cdiv(nbc, kBlock), kBlock, 0, ctx->stream>>>(nbc, d_bc, n, flags);
       if ((e = hipGetLastError()) != hipSuccess) { st = hip_fail(ctx, e, "bc_flag_kernel"); break; }
     }
-    A->vals32_valid = false;
+    csr_values_changed(A);
     dirichlet_kernel<<<(unsigned)cdiv(n, kBlock), kBlock, 0, ctx->stream>>>(
         n, A->row_offsets, A->cols, A->vals, b->d, flags, value);
     if ((e = hipGetLastError()) != hipSuccess) { st = hip_fail(ctx, e, "dirichlet_kernel"); break; }

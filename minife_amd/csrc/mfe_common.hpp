@@ -20,6 +20,7 @@ constexpr int kNumXcd = 8;         // MI355X: 8 XCDs, one L2 each
 constexpr int kSpmvRows = 256;     // rows per SpMV workgroup (one per thread)
 constexpr int kSpmvMaxNnz = 27 * kSpmvRows;  // LDS product slots per workgroup
 constexpr int kVecBlocks = 2048;   // fixed grid of the vector kernels
+constexpr int kSellMaxWidth = 27;  // longest row the SELL SpMV takes (27-point stencil)
 
 struct CgWorkspace;  // mfe_solve.hip
 
@@ -51,6 +52,21 @@ struct mfe_csr {
   int64_t num_cols = 0;
   // SpMV plan: largest nnz of any kSpmvRows-row block (-1 = not computed).
   int64_t max_block_nnz = -1;
+  // SELL-64 copy of the matrix for the SpMV (sliced ELLPACK: slices of 64
+  // rows, one row per lane, slot j of a slice stored as 64 consecutive
+  // entries), derived lazily from the CSR arrays and rebuilt after changes.
+  int spmv_format = MFE_SPMV_AUTO;
+  uint64_t plan_gen = 0;  // bumped whenever the arrays an SpMV launch reads move
+  bool sell_struct_valid = false;  // sell_off/len/cols match the CSR structure
+  bool sell_ok = false;            // every row fits kSellMaxWidth and the copy fits memory
+  bool sell_vals_valid = false, sell_vals32_valid = false;
+  int sell_max_len = 0;
+  int64_t sell_nslices = 0, sell_slots = 0;
+  int* sell_off = nullptr;           // nslices+1, in units of 64-entry slot columns
+  unsigned char* sell_len = nullptr;  // nslices*64 row lengths
+  int* sell_cols = nullptr;
+  double* sell_vals = nullptr;
+  float* sell_vals32 = nullptr;
 };
 
 struct mfe_vec {
@@ -68,6 +84,22 @@ int hip_fail(mfe_ctx* ctx, hipError_t e, const char* what);
 int plan_spmv(mfe_ctx* ctx, mfe_csr* A);
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Derived copies (FP32 values, SELL plan) are stale after these.
+inline void csr_values_changed(mfe_csr* A) {
+  A->vals32_valid = false;
+  A->sell_vals_valid = false;
+  A->sell_vals32_valid = false;
+}
+inline void csr_structure_changed(mfe_csr* A) {
+  csr_values_changed(A);
+  ++A->plan_gen;
+  A->max_block_nnz = -1;
+  A->sell_struct_valid = false;
+  A->sell_ok = false;
+}
+// Frees the SELL arrays (mfe_solve.hip).
+void sell_release(mfe_csr* A);
 
 }  // namespace mfe
 

@@ -125,23 +125,310 @@ __global__ __launch_bounds__(kBlock) void spmv_rowwise_kernel(
   }
 }
 
-// y = A x (values `vals`, FP64 or the FP32 copy) on stream; *tiles_out = the
-// number of partials written.
+// ------------------------------------------------------------ SELL-64 -------
+//
+// Slice s holds rows 64s..64s+63; its width w_s is its longest row, and slot
+// j < w_s of lane l sits at ((off[s] + j) * 64 + l).  Lane l owns row 64s+l:
+// its value and column loads are 64 consecutive entries per slot (coalesced,
+// no LDS), and on a stencil matrix the x gathers of one slot hit consecutive
+// addresses too.  Slots past a row's length hold value 0 and the row's own
+// index and are skipped in the sum, which runs over the row's entries in
+// stored column order: the same bits as the CSR kernels.  Four slices per
+// workgroup, so the partials match the CSR kernel's (one per 256 rows).
+
+constexpr int kSlice = kWave;
+constexpr int kSlicesPerBlock = kBlock / kSlice;
+
 template <typename T, bool ZERO_Y, int MODE>
-static int launch_spmv(const mfe_csr* A, const T* vals, const T* x, T* y, const T* bvec,
-                       T* partials, const int* done, hipStream_t stream, int* tiles_out) {
-  const int n = (int)A->nrows;
+__global__ __launch_bounds__(kBlock) void spmv_sell_kernel(
+    int n, int nslices, const int* __restrict__ soff, const unsigned char* __restrict__ slen,
+    const int* __restrict__ cols, const T* __restrict__ vals, const T* __restrict__ x,
+    T* __restrict__ y, const T* __restrict__ bvec, T* __restrict__ partials,
+    const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ T red[4];
+  const int t = (int)threadIdx.x;
+  const int slice = (int)blockIdx.x * kSlicesPerBlock + t / kSlice;
+  const int row = slice * kSlice + t % kSlice;
+  bool valid = false;
+  T sum = T(0), xr = T(0);
+  if (slice < nslices) {  // uniform per wave
+    valid = row < n;
+    const int rr = valid ? row : n - 1;
+    const T y0 = ZERO_Y ? y[rr] : T(0);
+    if (MODE == kDot) xr = x[rr];
+    const T br = MODE == kResid ? bvec[rr] : T(0);
+    const int o = soff[slice];
+    const int w = soff[slice + 1] - o;
+    const int len = slen[row];
+    if (ZERO_Y) sum = T(0) * y0;
+    if (w > 0) {
+      // All value/column loads first (slot index clamped into the slice, so
+      // every load is unconditional), then every x gather, then the sum.
+      const int64_t base = (int64_t)o * kSlice + t % kSlice;
+      int c[kSellMaxWidth];
+      T v[kSellMaxWidth], xv[kSellMaxWidth];
+#pragma unroll
+      for (int j = 0; j < kSellMaxWidth; ++j) {
+        const int64_t at = base + (int64_t)(j < w ? j : w - 1) * kSlice;
+        c[j] = cols[at];
+        v[j] = vals[at];
+      }
+#pragma unroll
+      for (int j = 0; j < kSellMaxWidth; ++j) xv[j] = x[c[j]];
+#pragma unroll
+      for (int j = 0; j < kSellMaxWidth; ++j)
+        if (j < len) sum += v[j] * xv[j];
+    }
+    if (MODE == kResid) sum = T(1) * br + T(-1) * sum;
+    if (valid) y[row] = sum;
+  }
+  if (MODE != kPlain) {
+    const T term = valid ? (MODE == kDot ? sum * xr : sum * sum) : T(0);
+    const T v0 = block_sum<T>(term, red);
+    if (t == 0) partials[blockIdx.x] = v0;
+  }
+}
+
+// Row lengths (clamped to 255) and slice widths.
+__global__ __launch_bounds__(kBlock) void sell_len_kernel(int n, int nslices,
+                                                          const int* __restrict__ ro,
+                                                          unsigned char* __restrict__ slen,
+                                                          int* __restrict__ width) {
+  const int row = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int slice = row / kSlice;
+  if (slice >= nslices) return;  // whole waves
+  const int len = row < n ? ro[row + 1] - ro[row] : 0;
+  slen[row] = (unsigned char)(len < 255 ? len : 255);
+  int m = len;
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, kWave));
+  if (row % kSlice == 0) width[slice] = m;
+}
+
+// Scatter the CSR rows into the slices: columns when scols is non-null,
+// values as V (double, or rounded to float) when svals is non-null.  Padding: value 0, column =
+// the row itself (rows past n use n-1), so every gather stays in bounds.
+template <typename V>
+__global__ __launch_bounds__(kBlock) void sell_fill_kernel(
+    int n, int nslices, const int* __restrict__ soff, const unsigned char* __restrict__ slen,
+    const int* __restrict__ ro, const int* __restrict__ cols, const double* __restrict__ vals,
+    int* __restrict__ scols, V* __restrict__ svals) {
+  const int row = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int slice = row / kSlice;
+  if (slice >= nslices) return;
+  const int o = soff[slice], w = soff[slice + 1] - o;
+  const int len = slen[row];
+  const int start = row < n ? ro[row] : 0;
+  const int pad = row < n ? row : n - 1;
+  for (int j = 0; j < w; ++j) {
+    const int64_t at = ((int64_t)o + j) * kSlice + row % kSlice;
+    const bool in = j < len;
+    if (scols) scols[at] = in ? cols[start + j] : pad;
+    if (svals) svals[at] = in ? (V)vals[start + j] : V(0);
+  }
+}
+
+void sell_release(mfe_csr* A) {
+  ++A->plan_gen;
+  hipFree(A->sell_off);
+  hipFree(A->sell_len);
+  hipFree(A->sell_cols);
+  hipFree(A->sell_vals);
+  hipFree(A->sell_vals32);
+  A->sell_off = nullptr;
+  A->sell_len = nullptr;
+  A->sell_cols = nullptr;
+  A->sell_vals = nullptr;
+  A->sell_vals32 = nullptr;
+  A->sell_struct_valid = A->sell_ok = false;
+  A->sell_vals_valid = A->sell_vals32_valid = false;
+  A->sell_nslices = A->sell_slots = 0;
+  A->sell_max_len = 0;
+}
+
+// The structure was examined and SELL does not apply (a long row, or no
+// memory for the copy): free the copy, keep the longest-row figure.
+static void sell_disable(mfe_csr* A) {
+  const int maxw = A->sell_max_len;
+  sell_release(A);
+  A->sell_struct_valid = true;
+  A->sell_max_len = maxw;
+}
+
+// hipMalloc that leaves no sticky error behind when memory is short (the
+// plan then falls back to the CSR kernels).
+template <typename P>
+static bool try_alloc(P** p, size_t bytes) {
+  if (hipMalloc(p, bytes) == hipSuccess) return true;
+  (void)hipGetLastError();
+  *p = nullptr;
+  return false;
+}
+
+// CSR FP32 copy of the values (round to nearest).
+__global__ void vals_to_f32_kernel(int64_t nnz, const double* __restrict__ v,
+                                   float* __restrict__ v32) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += stride)
+    v32[i] = (float)v[i];
+}
+
+// Builds what the next SpMV launches need: the SELL structure, and its FP64
+// and/or FP32 values; the CSR FP32 copy instead when SELL does not apply.
+// Synchronises the stream (host-side scan of the slice widths); never called
+// during graph capture.
+static int ensure_plan(mfe_ctx* ctx, const mfe_csr* Ac, bool need64, bool need32) {
+  mfe_csr* A = const_cast<mfe_csr*>(Ac);
+  hipStream_t s = ctx->stream;
+  const int64_t n = A->nrows;
+  if (A->spmv_format == MFE_SPMV_AUTO && !A->sell_struct_valid && n > 0) {
+    sell_release(A);
+    const int64_t ns = cdiv(n, kSlice);
+    int* d_width = nullptr;
+    bool ok = try_alloc(&A->sell_len, (size_t)(ns * kSlice)) &&
+              try_alloc(&A->sell_off, sizeof(int) * (size_t)(ns + 1)) &&
+              try_alloc(&d_width, sizeof(int) * (size_t)ns);
+    if (ok) {
+      sell_len_kernel<<<(unsigned)cdiv(ns * kSlice, kBlock), kBlock, 0, s>>>(
+          (int)n, (int)ns, A->row_offsets, A->sell_len, d_width);
+      MFE_LAUNCHED(ctx, "sell_len_kernel");
+      std::vector<int> width((size_t)ns), off((size_t)ns + 1);
+      MFE_CHECK(ctx, hipMemcpyAsync(width.data(), d_width, sizeof(int) * (size_t)ns,
+                                    hipMemcpyDeviceToHost, s));
+      MFE_CHECK(ctx, hipStreamSynchronize(s));
+      int64_t acc = 0;
+      int maxw = 0;
+      for (int64_t i = 0; i < ns; ++i) {
+        off[(size_t)i] = (int)acc;
+        acc += width[(size_t)i];
+        maxw = std::max(maxw, width[(size_t)i]);
+      }
+      off[(size_t)ns] = (int)acc;
+      A->sell_max_len = maxw;
+      A->sell_nslices = ns;
+      A->sell_slots = acc * kSlice;
+      ok = maxw <= kSellMaxWidth && acc < INT32_MAX &&
+           try_alloc(&A->sell_cols, sizeof(int) * (size_t)(A->sell_slots > 0 ? A->sell_slots : 1));
+      if (ok) {
+        MFE_CHECK(ctx, hipMemcpyAsync(A->sell_off, off.data(), sizeof(int) * (size_t)(ns + 1),
+                                      hipMemcpyHostToDevice, s));
+        sell_fill_kernel<double><<<(unsigned)cdiv(ns * kSlice, kBlock), kBlock, 0, s>>>(
+            (int)n, (int)ns, A->sell_off, A->sell_len, A->row_offsets, A->cols, A->vals,
+            A->sell_cols, nullptr);
+        MFE_LAUNCHED(ctx, "sell_fill_kernel (columns)");
+      }
+    }
+    hipFree(d_width);
+    A->sell_struct_valid = true;
+    if (ok) {
+      A->sell_ok = true;
+    } else {
+      sell_disable(A);
+    }
+  }
+  const bool sell = A->spmv_format == MFE_SPMV_AUTO && A->sell_ok && n > 0;
+  const unsigned fg = (unsigned)cdiv(A->sell_nslices * kSlice, kBlock);
+  if (sell && need64 && !A->sell_vals_valid) {
+    if (!A->sell_vals && !try_alloc(&A->sell_vals, sizeof(double) * (size_t)std::max<int64_t>(A->sell_slots, 1))) {
+      sell_disable(A);  // no room for the copy: CSR kernels
+      return ensure_plan(ctx, A, need64, need32);
+    }
+    sell_fill_kernel<double><<<fg, kBlock, 0, s>>>((int)n, (int)A->sell_nslices, A->sell_off,
+                                                   A->sell_len, A->row_offsets, A->cols, A->vals,
+                                                   nullptr, A->sell_vals);
+    MFE_LAUNCHED(ctx, "sell_fill_kernel<double>");
+    A->sell_vals_valid = true;
+  }
+  if (sell && need32 && !A->sell_vals32_valid) {
+    if (!A->sell_vals32 && !try_alloc(&A->sell_vals32, sizeof(float) * (size_t)std::max<int64_t>(A->sell_slots, 1))) {
+      sell_disable(A);
+      return ensure_plan(ctx, A, need64, need32);
+    }
+    sell_fill_kernel<float><<<fg, kBlock, 0, s>>>((int)n, (int)A->sell_nslices, A->sell_off,
+                                                  A->sell_len, A->row_offsets, A->cols, A->vals,
+                                                  nullptr, A->sell_vals32);
+    MFE_LAUNCHED(ctx, "sell_fill_kernel<float>");
+    A->sell_vals32_valid = true;
+  }
+  if (!sell && need32 && !A->vals32_valid) {
+    if (!A->vals32)
+      MFE_CHECK(ctx, hipMalloc(&A->vals32, sizeof(float) * (size_t)(A->nnz > 0 ? A->nnz : 1)));
+    if (A->nnz > 0) {
+      vals_to_f32_kernel<<<(unsigned)std::min<int64_t>(cdiv(A->nnz, kBlock), 65536), kBlock, 0,
+                           s>>>(A->nnz, A->vals, A->vals32);
+      MFE_LAUNCHED(ctx, "vals_to_f32_kernel");
+    }
+    A->vals32_valid = true;
+  }
+  return MFE_OK;
+}
+
+// What one SpMV launch reads: the SELL copy when the plan has one for T,
+// else the CSR arrays (FP64 values, or the CSR FP32 copy).
+template <typename T>
+struct SpmvOp {
+  const mfe_csr* A = nullptr;
+  uint64_t gen = 0;  // A->plan_gen when made: captured graphs are keyed on it
+  int n = 0;
+  const int* ro = nullptr;
+  const int* cols = nullptr;
+  const T* vals = nullptr;
+  bool stream_ok = false;  // CSR row-stream kernel applies (else row-per-thread)
+  int nslices = 0;
+  const int* soff = nullptr;
+  const unsigned char* slen = nullptr;
+  const int* scols = nullptr;
+  const T* svals = nullptr;  // non-null: SELL
+  const void* key() const { return svals ? (const void*)svals : (const void*)vals; }
+};
+
+template <typename T>
+static SpmvOp<T> spmv_op(const mfe_csr* A, bool allow_sell = true) {
+  SpmvOp<T> op;
+  op.A = A;
+  op.gen = A->plan_gen;
+  op.n = (int)A->nrows;
+  op.ro = A->row_offsets;
+  op.cols = A->cols;
+  op.stream_ok = A->max_block_nnz >= 0 && A->max_block_nnz <= kSpmvMaxNnz;
+  const bool sell = allow_sell && A->spmv_format == MFE_SPMV_AUTO && A->sell_ok;
+  if constexpr (std::is_same<T, double>::value) {
+    op.vals = A->vals;
+    if (sell && A->sell_vals_valid) op.svals = A->sell_vals;
+  } else {
+    op.vals = A->vals32;
+    if (sell && A->sell_vals32_valid) op.svals = A->sell_vals32;
+  }
+  if (op.svals) {
+    op.nslices = (int)A->sell_nslices;
+    op.soff = A->sell_off;
+    op.slen = A->sell_len;
+    op.scols = A->sell_cols;
+  }
+  return op;
+}
+
+// y = A x on stream; *tiles_out = the number of partials written (one per
+// 256 rows in every format).
+template <typename T, bool ZERO_Y, int MODE>
+static int launch_spmv(const SpmvOp<T>& op, const T* x, T* y, const T* bvec, T* partials,
+                       const int* done, hipStream_t stream, int* tiles_out) {
+  const int n = op.n;
   if (n == 0) {
     if (tiles_out) *tiles_out = 0;
     return 0;
   }
   const int tiles = (int)cdiv(n, kSpmvRows);
-  if (A->max_block_nnz >= 0 && A->max_block_nnz <= kSpmvMaxNnz) {
+  if (op.svals) {
+    spmv_sell_kernel<T, ZERO_Y, MODE><<<tiles, kBlock, 0, stream>>>(
+        n, op.nslices, op.soff, op.slen, op.scols, op.svals, x, y, bvec, partials, done);
+  } else if (op.stream_ok) {
     spmv_stream_kernel<T, ZERO_Y, MODE><<<tiles, kBlock, 0, stream>>>(
-        n, A->row_offsets, A->cols, vals, x, y, bvec, partials, done);
+        n, op.ro, op.cols, op.vals, x, y, bvec, partials, done);
   } else {
     spmv_rowwise_kernel<T, ZERO_Y, MODE><<<tiles, kBlock, 0, stream>>>(
-        n, A->row_offsets, A->cols, vals, x, y, bvec, partials, done);
+        n, op.ro, op.cols, op.vals, x, y, bvec, partials, done);
   }
   if (tiles_out) *tiles_out = tiles;
   return (int)hipGetLastError();
@@ -610,6 +897,7 @@ struct GraphSlot {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   const void *kA = nullptr, *kvals = nullptr, *kb = nullptr, *kx = nullptr;
+  uint64_t kgen = 0;
   unsigned kflags = 0;
   void drop() {
     if (exec) hipGraphExecDestroy(exec);
@@ -699,7 +987,7 @@ static inline void mark(OpTimer* tm, int slot) {
 
 // One CG iteration body (k = 1 when y_for_p == r, k >= 2 when y_for_p == p).
 template <typename T>
-static hipError_t enqueue_iteration(const mfe_csr* A, const T* vals, VecSet<T>& v, CgState* st,
+static hipError_t enqueue_iteration(const SpmvOp<T>& op, VecSet<T>& v, CgState* st,
                                     double* hist, T* x, const T* y_for_p, bool replay,
                                     hipStream_t s, OpTimer* tm = nullptr) {
   const int64_t n = v.n;
@@ -710,7 +998,7 @@ static hipError_t enqueue_iteration(const mfe_csr* A, const T* vals, VecSet<T>& 
   const int* done = &st->done;
   if constexpr (std::is_same<T, double>::value) {
     if (replay) {
-      launch_spmv<double, true, kPlain>(A, vals, v.p, v.Ap, nullptr, nullptr, done, s, &tiles);
+      launch_spmv<double, true, kPlain>(op, v.p, v.Ap, nullptr, nullptr, done, s, &tiles);
       mark(tm, MFE_T_MATVEC);
       cg_alpha_seq_kernel<<<1, kBlock, 0, s>>>(st, n, v.Ap, v.p, hist);
       mark(tm, MFE_T_DOT);
@@ -721,7 +1009,7 @@ static hipError_t enqueue_iteration(const mfe_csr* A, const T* vals, VecSet<T>& 
       return hipGetLastError();
     }
   }
-  launch_spmv<T, false, kDot>(A, vals, v.p, v.Ap, nullptr, v.partials, done, s, &tiles);
+  launch_spmv<T, false, kDot>(op, v.p, v.Ap, nullptr, v.partials, done, s, &tiles);
   mark(tm, MFE_T_MATVEC);
   const int nb2 = (int)cdiv(tiles, kStage);
   reduce_stage_kernel<T><<<nb2, kBlock, 0, s>>>(v.partials, tiles, 1, v.red2, done);
@@ -739,7 +1027,7 @@ static hipError_t enqueue_iteration(const mfe_csr* A, const T* vals, VecSet<T>& 
 // Unpreconditioned CG (cg_solve.hpp:69-229) on vectors of type T from the
 // initial x.  Leaves the final state in w->h_st.
 template <typename T>
-static int run_cg(mfe_ctx* ctx, const mfe_csr* A, const T* vals, const T* b, T* x, int max_iter,
+static int run_cg(mfe_ctx* ctx, const SpmvOp<T>& op, const T* b, T* x, int max_iter,
                   double tol, bool replay, bool eager, CgWorkspace* w, VecSet<T>& v,
                   GraphSlot& slot, const void* key_b, const void* key_x, unsigned key_flags,
                   double* times = nullptr) {
@@ -756,7 +1044,7 @@ static int run_cg(mfe_ctx* ctx, const mfe_csr* A, const T* vals, const T* b, T* 
     mark(tm, MFE_T_WAXPY);
     // The reference's Ap starts as a zero vector, so its 0*Ap[row] is +0.0:
     // start the sums at +0.0 instead of reading the workspace.
-    int e = launch_spmv<T, false, kPlain>(A, vals, v.p, v.Ap, nullptr, nullptr, nullptr, s, nullptr);
+    int e = launch_spmv<T, false, kPlain>(op, v.p, v.Ap, nullptr, nullptr, nullptr, s, nullptr);
     if (e) return hip_fail(ctx, (hipError_t)e, "cg prologue spmv");
     mark(tm, MFE_T_MATVEC);
     waxpby_kernel<T><<<vg, kBlock, 0, s>>>(n, T(1), b, T(-1), v.Ap, v.r);
@@ -773,12 +1061,12 @@ static int run_cg(mfe_ctx* ctx, const mfe_csr* A, const T* vals, const T* b, T* 
     mark(tm, MFE_T_DOT);
     MFE_LAUNCHED(ctx, "cg prologue");
     // Iteration 1: p = 1*r + 0*r.
-    hipError_t he = enqueue_iteration<T>(A, vals, v, w->st, w->hist, x, v.r, replay, s, tm);
+    hipError_t he = enqueue_iteration<T>(op, v, w->st, w->hist, x, v.r, replay, s, tm);
     if (he != hipSuccess) return hip_fail(ctx, he, "cg iteration 1");
     if (max_iter > 1) {
       if (eager) {
         for (int k = 2; k <= max_iter; ++k) {
-          he = enqueue_iteration<T>(A, vals, v, w->st, w->hist, x, v.p, replay, s, tm);
+          he = enqueue_iteration<T>(op, v, w->st, w->hist, x, v.p, replay, s, tm);
           if (he != hipSuccess) return hip_fail(ctx, he, "cg iteration");
           if (k % kGraphChunk == 0) {
             MFE_CHECK(ctx, hipMemcpyAsync(w->h_st, w->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
@@ -787,14 +1075,15 @@ static int run_cg(mfe_ctx* ctx, const mfe_csr* A, const T* vals, const T* b, T* 
           }
         }
       } else {
-        const bool reuse = slot.exec && slot.kA == A && slot.kvals == vals && slot.kb == key_b &&
+        const bool reuse = slot.exec && slot.kA == op.A && slot.kgen == op.gen &&
+                           slot.kvals == op.key() && slot.kb == key_b &&
                            slot.kx == key_x && slot.kflags == key_flags;
         if (!reuse) {
           slot.drop();
           MFE_CHECK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
           hipError_t ce = hipSuccess;
           for (int c = 0; c < kGraphChunk && ce == hipSuccess; ++c)
-            ce = enqueue_iteration<T>(A, vals, v, w->st, w->hist, x, v.p, replay, s);
+            ce = enqueue_iteration<T>(op, v, w->st, w->hist, x, v.p, replay, s);
           hipGraph_t g = nullptr;
           hipError_t ee = hipStreamEndCapture(s, &g);
           if (ce != hipSuccess) {
@@ -804,8 +1093,9 @@ static int run_cg(mfe_ctx* ctx, const mfe_csr* A, const T* vals, const T* b, T* 
           if (ee != hipSuccess) return hip_fail(ctx, ee, "hipStreamEndCapture");
           slot.graph = g;
           MFE_CHECK(ctx, hipGraphInstantiate(&slot.exec, g, nullptr, nullptr, 0));
-          slot.kA = A;
-          slot.kvals = vals;
+          slot.kA = op.A;
+          slot.kgen = op.gen;
+          slot.kvals = op.key();
           slot.kb = key_b;
           slot.kx = key_x;
           slot.kflags = key_flags;
@@ -828,27 +1118,6 @@ static int run_cg(mfe_ctx* ctx, const mfe_csr* A, const T* vals, const T* b, T* 
   MFE_CHECK(ctx, hipMemcpyAsync(w->h_st, w->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
   MFE_CHECK(ctx, hipStreamSynchronize(s));
   if (tm) tm->accumulate(times);
-  return MFE_OK;
-}
-
-__global__ void vals_to_f32_kernel(int64_t nnz, const double* __restrict__ v,
-                                   float* __restrict__ v32) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += stride)
-    v32[i] = (float)v[i];
-}
-
-// FP32 copy of A's values (round to nearest), made once per set of values.
-static int ensure_fp32(mfe_ctx* ctx, const mfe_csr* Ac) {
-  mfe_csr* A = const_cast<mfe_csr*>(Ac);
-  if (A->vals32_valid) return MFE_OK;
-  if (!A->vals32) MFE_CHECK(ctx, hipMalloc(&A->vals32, sizeof(float) * (size_t)(A->nnz > 0 ? A->nnz : 1)));
-  if (A->nnz > 0) {
-    vals_to_f32_kernel<<<(unsigned)std::min<int64_t>(cdiv(A->nnz, kBlock), 65536), kBlock, 0,
-                         ctx->stream>>>(A->nnz, A->vals, A->vals32);
-    MFE_LAUNCHED(ctx, "vals_to_f32_kernel");
-  }
-  A->vals32_valid = true;
   return MFE_OK;
 }
 
@@ -888,13 +1157,44 @@ static int read_scalar(mfe_ctx* ctx, const double* d, double* out) {
   return MFE_OK;
 }
 
+int mfe_spmv_plan(mfe_ctx* ctx, mfe_csr* A, int format, int64_t info[4]) {
+  if (!ctx || !A || (format != MFE_SPMV_AUTO && format != MFE_SPMV_CSR)) return MFE_BAD_ARG;
+  MFE_CHECK(ctx, hipSetDevice(ctx->device));
+  if (format != A->spmv_format) {
+    MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    sell_release(A);
+    A->spmv_format = format;
+  }
+  int st;
+  if ((st = ensure_plan(ctx, A, true, false))) return st;
+  if (info) {
+    const SpmvOp<double> op = spmv_op<double>(A);
+    info[0] = op.svals ? 1 : (op.stream_ok ? 0 : 2);
+    info[1] = op.svals ? A->sell_slots : A->nnz;
+    info[2] = op.svals ? A->sell_nslices : cdiv(A->nrows, kSpmvRows);
+    int64_t longest = A->sell_max_len;
+    if (!A->sell_struct_valid) {  // CSR-only plan: measure on the host
+      std::vector<int> ro((size_t)A->nrows + 1);
+      MFE_CHECK(ctx, hipMemcpyAsync(ro.data(), A->row_offsets, sizeof(int) * ro.size(),
+                                    hipMemcpyDeviceToHost, ctx->stream));
+      MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+      longest = 0;
+      for (int64_t r = 0; r < A->nrows; ++r) longest = std::max<int64_t>(longest, ro[r + 1] - ro[r]);
+    }
+    info[3] = longest;
+  }
+  MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return MFE_OK;
+}
+
 int mfe_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y) {
   if (!ctx || !A) return MFE_BAD_ARG;
   int st;
   if ((st = check_vec(ctx, x, A->nrows, "matvec x")) || (st = check_vec(ctx, y, A->nrows, "matvec y")))
     return st;
-  int e = launch_spmv<double, true, kPlain>(A, A->vals, x->d, y->d, nullptr, nullptr, nullptr,
-                                            ctx->stream, nullptr);
+  if ((st = ensure_plan(ctx, A, true, false))) return st;
+  int e = launch_spmv<double, true, kPlain>(spmv_op<double>(A), x->d, y->d, nullptr, nullptr,
+                                            nullptr, ctx->stream, nullptr);
   if (e) return hip_fail(ctx, (hipError_t)e, "spmv");
   return MFE_OK;
 }
@@ -907,9 +1207,10 @@ int mfe_matvec_and_dot(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec
     return st;
   const int64_t tiles = cdiv(A->nrows, kSpmvRows);
   if ((st = ensure_partials(ctx, tiles > 0 ? tiles : 1))) return st;
+  if ((st = ensure_plan(ctx, A, true, false))) return st;
   int nt = 0;
-  int e = launch_spmv<double, true, kDot>(A, A->vals, x->d, y->d, nullptr, ctx->d_partials,
-                                          nullptr, ctx->stream, &nt);
+  int e = launch_spmv<double, true, kDot>(spmv_op<double>(A), x->d, y->d, nullptr,
+                                          ctx->d_partials, nullptr, ctx->stream, &nt);
   if (e) return hip_fail(ctx, (hipError_t)e, "spmv_dot");
   if (nt == 0) {
     *result = 0;
@@ -929,11 +1230,11 @@ int mfe_matvec_fp32(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y
   if (n == 0) return MFE_OK;
   CgWorkspace* w = nullptr;
   if ((st = ws_get(ctx, 1, &w)) || (st = vecset_prepare<float>(ctx, w->v32, n, w->g32))) return st;
-  if ((st = ensure_fp32(ctx, A))) return st;
+  if ((st = ensure_plan(ctx, A, false, true))) return st;
   const unsigned vg = vec_grid(n);
   to_f32_kernel<<<vg, kBlock, 0, ctx->stream>>>(n, x->d, 1.0, w->v32.p);
-  int e = launch_spmv<float, false, kPlain>(A, A->vals32, w->v32.p, w->v32.Ap, nullptr, nullptr,
-                                            nullptr, ctx->stream, nullptr);
+  int e = launch_spmv<float, false, kPlain>(spmv_op<float>(A), w->v32.p, w->v32.Ap, nullptr,
+                                            nullptr, nullptr, ctx->stream, nullptr);
   if (e) return hip_fail(ctx, (hipError_t)e, "spmv fp32");
   to_f64_kernel<<<vg, kBlock, 0, ctx->stream>>>(n, w->v32.Ap, y->d);
   MFE_LAUNCHED(ctx, "to_f64_kernel");
@@ -942,31 +1243,45 @@ int mfe_matvec_fp32(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y
 
 int mfe_time_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y, int variant,
                     int reps, double* avg_ms) {
-  if (!ctx || !A || !avg_ms || reps <= 0 || variant < 0 || variant > 2) return MFE_BAD_ARG;
+  if (!ctx || !A || !avg_ms || reps <= 0 || variant < 0 || variant > 4) return MFE_BAD_ARG;
   int st;
   if ((st = check_vec(ctx, x, A->nrows, "matvec x")) || (st = check_vec(ctx, y, A->nrows, "matvec y")))
     return st;
   const int64_t n = A->nrows;
   const int64_t tiles = cdiv(n, kSpmvRows);
   if ((st = ensure_partials(ctx, tiles > 0 ? tiles : 1))) return st;
+  const bool f32 = variant == 2 || variant == 4;
+  const bool csr_only = variant >= 3;
+  if ((st = ensure_plan(ctx, A, !f32, f32))) return st;
+  SpmvOp<double> op64 = spmv_op<double>(A, !csr_only);
+  SpmvOp<float> op32 = spmv_op<float>(A, !csr_only);
   CgWorkspace* w = nullptr;
-  if (variant == 2) {
+  if (f32) {
     if ((st = ws_get(ctx, 1, &w)) || (st = vecset_prepare<float>(ctx, w->v32, n, w->g32))) return st;
-    if ((st = ensure_fp32(ctx, A))) return st;
+    if (csr_only && !A->vals32_valid) {  // CSR FP32 copy for the comparison run
+      mfe_csr* Am = const_cast<mfe_csr*>(A);
+      const int keep = Am->spmv_format;
+      Am->spmv_format = MFE_SPMV_CSR;
+      st = ensure_plan(ctx, Am, false, true);
+      Am->spmv_format = keep;
+      if (st) return st;
+      op32 = spmv_op<float>(A, false);
+    }
     to_f32_kernel<<<vec_grid(n), kBlock, 0, ctx->stream>>>(n, x->d, 1.0, w->v32.p);
+    MFE_LAUNCHED(ctx, "to_f32_kernel");
   }
   MFE_CHECK(ctx, hipEventRecord(ctx->t_start, ctx->stream));
   for (int i = 0; i < reps; ++i) {
     int e;
     if (variant == 0)
-      e = launch_spmv<double, true, kPlain>(A, A->vals, x->d, y->d, nullptr, nullptr, nullptr,
+      e = launch_spmv<double, true, kPlain>(op64, x->d, y->d, nullptr, nullptr, nullptr,
                                             ctx->stream, nullptr);
-    else if (variant == 1)
-      e = launch_spmv<double, false, kDot>(A, A->vals, x->d, y->d, nullptr, ctx->d_partials,
-                                           nullptr, ctx->stream, nullptr);
+    else if (!f32)
+      e = launch_spmv<double, false, kDot>(op64, x->d, y->d, nullptr, ctx->d_partials, nullptr,
+                                           ctx->stream, nullptr);
     else
-      e = launch_spmv<float, false, kDot>(A, A->vals32, w->v32.p, w->v32.Ap, nullptr,
-                                          w->v32.partials, nullptr, ctx->stream, nullptr);
+      e = launch_spmv<float, false, kDot>(op32, w->v32.p, w->v32.Ap, nullptr, w->v32.partials,
+                                          nullptr, ctx->stream, nullptr);
     if (e) return hip_fail(ctx, (hipError_t)e, "spmv");
   }
   MFE_CHECK(ctx, hipEventRecord(ctx->t_stop, ctx->stream));
@@ -1036,7 +1351,8 @@ int mfe_cg_solve(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x, i
   *num_iters = 0;
   double op_times[5] = {0, 0, 0, 0, 0};
   MFE_CHECK(ctx, hipEventRecord(ctx->t_start, ctx->stream));
-  st = run_cg<double>(ctx, A, A->vals, b->d, x->d, max_iter, tol, replay, eager, w, w->v64, w->g64,
+  if ((st = ensure_plan(ctx, A, true, false))) return st;
+  st = run_cg<double>(ctx, spmv_op<double>(A), b->d, x->d, max_iter, tol, replay, eager, w, w->v64, w->g64,
                       b, x, flags & MFE_CG_REPLAY, times ? op_times : nullptr);
   if (st) return st;
   MFE_CHECK(ctx, hipEventRecord(ctx->t_stop, ctx->stream));
@@ -1088,7 +1404,9 @@ int mfe_cg_solve_mixed(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec
     w->f_n = n;
   }
   MFE_CHECK(ctx, hipEventRecord(ctx->t_start, s));
-  if ((st = ensure_fp32(ctx, A))) return st;
+  if ((st = ensure_plan(ctx, A, true, true))) return st;
+  const SpmvOp<double> op64 = spmv_op<double>(A);
+  const SpmvOp<float> op32 = spmv_op<float>(A);
   if (n == 0) return MFE_OK;
   const unsigned vg = vec_grid(n);
   const int tiles = (int)cdiv(n, kSpmvRows);
@@ -1098,7 +1416,7 @@ int mfe_cg_solve_mixed(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec
   VecSet<double>& v = w->v64;
   // r = 1*b + (-1)*(A x) in FP64 with the partials of r.r, fused in the SpMV.
   auto residual = [&](double* rnorm) -> int {
-    int e = launch_spmv<double, false, kResid>(A, A->vals, x->d, v.r, b->d, v.partials, nullptr,
+    int e = launch_spmv<double, false, kResid>(op64, x->d, v.r, b->d, v.partials, nullptr,
                                                s, nullptr);
     if (e) return hip_fail(ctx, (hipError_t)e, "residual spmv");
     finalize_kernel<<<1, kBlock, 0, s>>>(v.partials, tiles, 1, ctx->d_scalars);
@@ -1128,7 +1446,7 @@ int mfe_cg_solve_mixed(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec
     to_f32_kernel<<<vg, kBlock, 0, s>>>(n, v.r, 1.0 / rn, w->f_rhs);
     MFE_CHECK(ctx, hipMemsetAsync(w->f_x, 0, sizeof(float) * (size_t)n, s));
     MFE_LAUNCHED(ctx, "to_f32_kernel");
-    if ((st = run_cg<float>(ctx, A, A->vals32, w->f_rhs, w->f_x, max_inner, tol_k, false,
+    if ((st = run_cg<float>(ctx, op32, w->f_rhs, w->f_x, max_inner, tol_k, false,
                             false, w, w->v32, w->g32, w->f_rhs, w->f_x, 0)))
       return st;
     const CgState hs = *w->h_st;

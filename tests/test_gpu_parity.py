@@ -211,12 +211,97 @@ def test_long_rows_take_the_rowwise_path(mfe, ctx, oracle_mod):
                            if l <= n else np.sort(rng.integers(0, n, int(l))) for l in lens])
     vals = rng.standard_normal(cols.shape[0])
     A = mfe.CSRMatrix.from_host(ro, cols, vals, ctx=ctx)
+    assert mfe.spmv_plan(A)["kernel"] == "csr-rowwise"  # rows too long for SELL
     xh = rng.standard_normal(n)
     x = mfe.Vector.from_numpy(xh, ctx)
     y = mfe.Vector(n, ctx)
     mfe.matvec(A, x, y)
     Ao = oracle_mod.CSR(np.arange(n, dtype=np.int32), ro, cols.astype(np.int32), vals)
     assert bits_equal(y.to_numpy(), oracle_mod.matvec(Ao, xh))
+
+
+def _ragged_csr(rng, n, maxlen=27):
+    """Random CSR with row lengths 0..maxlen (some empty), sorted columns."""
+    lens = rng.integers(0, maxlen + 1, n)
+    lens[rng.choice(n, size=max(1, n // 10), replace=False)] = 0
+    ro = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    cols = np.concatenate([np.sort(rng.choice(n, size=int(l), replace=False)) for l in lens]
+                          + [np.zeros(0, np.int64)]).astype(np.int32)
+    vals = rng.standard_normal(cols.shape[0])
+    vals[rng.random(cols.shape[0]) < 0.05] = 0.0  # zero products, sign of zero matters
+    return ro, cols, vals
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 1000, 4099])
+def test_sell_and_csr_kernels_bit_exact_on_ragged_rows(mfe, ctx, oracle_mod, n):
+    """Both SpMV formats equal the oracle bit for bit on ragged rows, empty
+    rows (whose result is 0*y: -0.0 for negative y) and row counts that are
+    not a multiple of the 64-row slice."""
+    say(f"case ragged spmv n={n}")
+    rng = np.random.default_rng(n)
+    ro, cols, vals = _ragged_csr(rng, n)
+    Ao = oracle_mod.CSR(np.arange(n, dtype=np.int32), ro, cols, vals)
+    xh = rng.standard_normal(n)
+    yh = -np.abs(rng.standard_normal(n)) - 1.0
+    want = oracle_mod.matvec(Ao, xh, yh.copy())
+    want32 = oracle_mod.matvec_f32(Ao, xh)
+    for fmt, kernel in [(mfe.SPMV_AUTO, "sell64"), (mfe.SPMV_CSR, "csr-stream")]:
+        A = mfe.CSRMatrix.from_host(ro, cols, vals, ctx=ctx)
+        info = mfe.spmv_plan(A, fmt)
+        assert info["kernel"] == kernel and info["longest_row"] == int(np.diff(ro).max())
+        x = mfe.Vector.from_numpy(xh, ctx)
+        y = mfe.Vector.from_numpy(yh, ctx)
+        mfe.matvec(A, x, y)
+        assert bits_equal(y.to_numpy(), want), kernel
+        y2 = mfe.Vector(n, ctx)
+        d = mfe.matvec_and_dot(A, x, y2)
+        assert bits_equal(y2.to_numpy(), oracle_mod.matvec(Ao, xh))
+        mfe.matvec_fp32(A, x, y2)
+        assert bits_equal(y2.to_numpy(), want32), kernel
+
+
+def test_sell_plan_follows_value_and_structure_changes(mfe, ctx, oracle_mod):
+    """The SELL copy is rebuilt after the values change (element loop,
+    Dirichlet) and after a new structure is generated into the same matrix."""
+    say("case sell plan rebuilds")
+    dims = (9, 7, 8)
+    mesh, A, b, _ = build_stages(mfe, ctx, dims)
+    P = oracle_mod.build_problem(*dims)
+    xh = np.random.default_rng(2).standard_normal(A.nrows)
+    x = mfe.Vector.from_numpy(xh, ctx)
+    y = mfe.Vector(A.nrows, ctx)
+    assert mfe.spmv_plan(A)["kernel"] == "sell64"
+    mfe.matvec(A, x, y)
+    assert bits_equal(y.to_numpy(), oracle_mod.matvec(P["A"], xh))
+    mfe.impose_dirichlet(0.0, A, b, dims[0] + 1, dims[1] + 1, dims[2] + 1, [3, 4, 50])
+    Ao = P["A"].copy()
+    oracle_mod.impose_dirichlet(0.0, Ao, P["b"].copy(), np.array([3, 4, 50], np.int32))
+    mfe.matvec(A, x, y)
+    assert bits_equal(y.to_numpy(), oracle_mod.matvec(Ao, xh))
+    mesh2 = mfe.simple_mesh_description([0, 5, 0, 6, 0, 4])
+    assert mfe.generate_matrix_structure(mesh2, A) == 0
+    b2 = mfe.Vector(A.nrows, ctx)
+    mfe.assemble_FE_data(mesh2, A, b2)
+    P2 = oracle_mod.build_problem(5, 6, 4)
+    xh2 = np.random.default_rng(3).standard_normal(A.nrows)
+    y2 = mfe.Vector(A.nrows, ctx)
+    mfe.matvec(A, mfe.Vector.from_numpy(xh2, ctx), y2)
+    assert bits_equal(y2.to_numpy(), oracle_mod.matvec(P2["A_prebc"], xh2))
+
+
+def test_cg_identical_across_spmv_formats(mfe, ctx):
+    """Both formats produce the same per-256-row partials, so the fast CG
+    trajectory and solution are bit-identical."""
+    say("case cg formats")
+    out = []
+    for fmt in (mfe.SPMV_CSR, mfe.SPMV_AUTO):
+        mesh, A, b, _ = build_stages(mfe, ctx, (24, 20, 22))
+        mfe.spmv_plan(A, fmt)
+        x = mfe.Vector(A.nrows, ctx)
+        r = mfe.cg_solve(A, b, x, max_iter=60, tolerance=0.0, history=True)
+        out.append((x.to_numpy(), r.history))
+    assert bits_equal(out[0][0], out[1][0])
+    assert bits_equal(out[0][1], out[1][1])
 
 
 @pytest.mark.parametrize("dims", [(10, 10, 10), (40, 33, 29)])
