@@ -42,6 +42,7 @@ typedef enum mfe_status {
 typedef struct mfe_ctx mfe_ctx;
 typedef struct mfe_csr mfe_csr;
 typedef struct mfe_vec mfe_vec;
+typedef struct mfe_batch mfe_batch;
 
 /* cg_solve flags */
 #define MFE_CG_REPLAY       0x1u /* sequential dots: reference's exact trajectory */
@@ -216,6 +217,33 @@ int mfe_matvec_fp32(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y
  * and 2 on the CSR kernels whatever A's plan (for comparison). */
 int mfe_time_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y, int variant,
                     int reps, double* avg_ms);
+
+/* ---- batched boxes --------------------------------------------------------- */
+/* Many independent miniFE box problems (no reference counterpart: the
+ * reference solves one box per run).  Problem p is the unit cube meshed with
+ * dims[3p..3p+2] = (nx, ny, nz) hex8 elements, unit conductivity, the
+ * reference's source and Dirichlet faces.  mfe_batch_solve assembles every
+ * problem and solves it with FP64 CG to ||r|| <= rel_tol * ||b|| in ONE
+ * cooperative persistent launch; each assembled operator is bit-identical to
+ * the reference's (generate_matrix_structure + assemble_FE_data + both
+ * impose_dirichlet passes, driver.hpp:181-218).  Needs cooperative launch and
+ * at least one resident workgroup per problem (else MFE_UNSUPPORTED). */
+int mfe_batch_create(mfe_ctx* ctx, int nprob, const int* dims, mfe_batch** out);
+void mfe_batch_destroy(mfe_batch* batch);
+/* p = -1: {problems, workgroups, total rows, stored SELL entries, total
+ * non-zeros, 0}; p >= 0: {rows, non-zeros, elements, workgroups, first
+ * workgroup, stored SELL entries} of problem p. */
+int mfe_batch_info(const mfe_batch* batch, int p, int64_t info[6]);
+/* iters[nprob], rel_residual[nprob] (recursive ||r||/||b||) and the launch's
+ * GPU time in seconds (all nullable).  MFE_BREAKDOWN if a problem hit
+ * non-positive p.Ap; MFE_HIP_ERROR if a problem barrier waited past its
+ * 10 s limit (the launch is then abandoned by every workgroup). */
+int mfe_batch_solve(mfe_ctx* ctx, mfe_batch* batch, int max_iter, double rel_tol, int* iters,
+                    double* rel_residual, double* seconds);
+/* Problem p's assembled operator as CSR (row_offsets[n+1], cols[nnz],
+ * vals[nnz]), its right-hand side b[n] and solution x[n]; any may be NULL. */
+int mfe_batch_download(mfe_ctx* ctx, const mfe_batch* batch, int p, int* row_offsets, int* cols,
+                       double* vals, double* b, double* x);
 
 /* ---- verification (host) -------------------------------------------------- */
 /* soln(x, y, z, max_p, max_q), analytic_soln.hpp:80-112. */

@@ -48,7 +48,8 @@ EXPORTS = [
     "mfe_impose_dirichlet", "mfe_make_local_matrix", "mfe_spmv_plan",
     "mfe_matvec", "mfe_matvec_and_dot", "mfe_dot", "mfe_dot_replay", "mfe_waxpby",
     "mfe_cg_solve", "mfe_cg_solve_mixed", "mfe_matvec_fp32", "mfe_time_matvec", "mfe_soln",
-    "mfe_verify_solution",
+    "mfe_verify_solution", "mfe_batch_create", "mfe_batch_destroy", "mfe_batch_info",
+    "mfe_batch_solve", "mfe_batch_download",
 ]
 
 
@@ -144,6 +145,12 @@ def lib() -> ctypes.CDLL:
         "mfe_time_matvec": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
                              ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "mfe_soln": ([ctypes.c_double] * 3 + [ctypes.c_int, ctypes.c_int], ctypes.c_double),
+        "mfe_batch_create": ([_vp, ctypes.c_int, _vp, ctypes.POINTER(_vp)], ctypes.c_int),
+        "mfe_batch_destroy": ([_vp], None),
+        "mfe_batch_info": ([_vp, ctypes.c_int, _vp], ctypes.c_int),
+        "mfe_batch_solve": ([_vp, _vp, ctypes.c_int, ctypes.c_double, _vp, _vp,
+                             ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "mfe_batch_download": ([_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
         "mfe_verify_solution": ([_vp, _int6, _vp, ctypes.c_double, ctypes.c_int,
                                  ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     }
@@ -530,6 +537,63 @@ def element_gradient(mesh: simple_mesh_description, lam: Vector, utilde: Vector,
     """grad[e] = -lambda_e^T K_e(1) utilde_e (the adjoint element gradient)."""
     lam.ctx.check(lib().mfe_element_gradient(lam.ctx.h, _box6(mesh.global_box), lam.h, utilde.h,
                                              grad.h), "element_gradient")
+
+
+class Batch:
+    """Many independent box problems (dims: (nprob, 3) elements per side),
+    assembled and solved by FP64 CG in one cooperative persistent launch."""
+
+    def __init__(self, dims, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        d = np.ascontiguousarray(np.asarray(dims, dtype=np.int32).reshape(-1, 3))
+        self.dims = d
+        self.nprob = d.shape[0]
+        h = _vp()
+        self.ctx.check(lib().mfe_batch_create(self.ctx.h, int(self.nprob), _ptr(d), ctypes.byref(h)),
+                       "batch_create")
+        self.h = h
+
+    def info(self, p: int = -1) -> dict:
+        a = np.zeros(6, np.int64)
+        self.ctx.check(lib().mfe_batch_info(self.h, int(p), _ptr(a)), "batch_info")
+        if p < 0:
+            keys = ["problems", "workgroups", "rows", "stored", "nnz"]
+        else:
+            keys = ["rows", "nnz", "elements", "workgroups", "first_workgroup", "stored"]
+        return {k: int(v) for k, v in zip(keys, a)}
+
+    def solve(self, max_iter: int = 5000, rel_tol: float = 1e-10) -> dict:
+        iters = np.zeros(self.nprob, np.int32)
+        rel = np.zeros(self.nprob, np.float64)
+        sec = ctypes.c_double(0)
+        st = lib().mfe_batch_solve(self.ctx.h, self.h, int(max_iter), float(rel_tol), _ptr(iters),
+                                   _ptr(rel), ctypes.byref(sec))
+        if st not in (MFE_OK, MFE_BREAKDOWN):
+            self.ctx.check(st, "batch_solve")
+        return dict(iters=iters, rel_residual=rel, seconds=sec.value, status=st)
+
+    def download(self, p: int, matrix: bool = True) -> dict:
+        inf = self.info(p)
+        n, nnz = inf["rows"], inf["nnz"]
+        b = np.empty(n, np.float64)
+        x = np.empty(n, np.float64)
+        ro = np.empty(n + 1, np.int32) if matrix else None
+        cols = np.empty(nnz, np.int32) if matrix else None
+        vals = np.empty(nnz, np.float64) if matrix else None
+        self.ctx.check(lib().mfe_batch_download(self.ctx.h, self.h, int(p), _ptr(ro), _ptr(cols),
+                                                _ptr(vals), _ptr(b), _ptr(x)), "batch_download")
+        return dict(row_offsets=ro, cols=cols, vals=vals, b=b, x=x)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mfe_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def soln(x: float, y: float, z: float, max_p: int = 300, max_q: int = 300) -> float:

@@ -1,0 +1,652 @@
+// Batched miniFE: many independent box problems of uneven size assembled and
+// solved by FP64 CG in ONE persistent, cooperative launch.
+//
+// Each problem owns a contiguous range of the grid's workgroups, sized by its
+// expected work (non-zeros x CG iterations, the latter ~ the box's extent).
+// The workgroups of a problem meet at a problem-local barrier (an arrival
+// counter plus a generation word, agent-scope release/acquire), so problems
+// never wait for each other.  The cooperative launch guarantees that every
+// workgroup is resident, and every wait is bounded by a wall-clock limit: a
+// wait that exceeds it raises the batch's abort flag and all workgroups leave.
+//
+// Per problem (same arithmetic as the single-box path, so the assembled
+// operator is bit-identical to the reference's):
+//   1. element matrices K_e (k = 1) and source vectors F_e into scratch (SoA);
+//   2. one thread per row: the row's entries summed in ascending element id,
+//      both Dirichlet passes (u = 0 on bc_rows_0, then u = 1 on bc_rows_1,
+//      row-local because the BC sets are faces of the box), stored straight
+//      into the problem's SELL-64 slices; x = 0, r = b;
+//   3. CG (cg_solve.hpp:69-229 with the fast, tree-reduced dots): p update,
+//      barrier, SpMV + p.Ap partials, barrier, x/r update + r.r partials,
+//      barrier.  Partials alternate between two buffers, so a workgroup that
+//      runs ahead never overwrites what a slower one is still summing.  Every
+//      workgroup sums a problem's partials in the same order, so all of them
+//      take the same scalars and leave the loop at the same iteration.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "mfe_common.hpp"
+#include "mfe_hex8.hpp"
+
+namespace mfe {
+
+struct BatchProb {
+  int Nx, Ny, Nz;  // elements per side
+  int n;           // rows = (Nx+1)(Ny+1)(Nz+1)
+  int ne;          // elements
+  int nslices;     // ceil(n / 64)
+  int b0, nb;      // first workgroup, workgroup count
+  int64_t row0;    // base of the problem's rows in the vectors
+  int64_t soff0;   // base of its nslices+1 slice offsets
+  int64_t slot0;   // base of its SELL entries
+  int64_t elem0;   // base of its element scratch (44 doubles per element)
+  int64_t part0;   // base of its 2*nb partials
+};
+
+struct BatchState {  // per problem, zeroed before each launch
+  unsigned count, gen;
+  int iters, status;  // status: 0 converged or max_iter, 1 breakdown, 2 aborted
+  double normr, tol, bnorm, pad;
+};
+
+constexpr double kBatchWaitSeconds = 10.0;
+
+// Problem-local barrier.  Every thread fences (release of its own stores),
+// thread 0 arrives and waits for the generation to move, every thread fences
+// again (acquire).  Returns false when the batch was aborted.
+__device__ bool prob_barrier(BatchState* st, int nb, int* abort_flag, uint64_t limit_ticks) {
+  __threadfence();
+  __syncthreads();
+  __shared__ int s_ok;
+  if (threadIdx.x == 0) {
+    const unsigned g = __hip_atomic_load(&st->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned arrived =
+        __hip_atomic_fetch_add(&st->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+    int ok = 1;
+    if (arrived == (unsigned)nb) {
+      __hip_atomic_store(&st->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&st->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      const uint64_t t0 = wall_clock64();
+      while (__hip_atomic_load(&st->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+          ok = 0;
+          break;
+        }
+        if (wall_clock64() - t0 > limit_ticks) {
+          __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
+    s_ok = ok;
+  }
+  __syncthreads();
+  __threadfence();
+  return s_ok != 0;
+}
+
+// Sum of a problem's nb partials in a fixed order (the same in every workgroup).
+__device__ double sum_partials_all(const double* pp, int nb, double* red) {
+  double s = 0.0;
+  for (int i = (int)threadIdx.x; i < nb; i += kBlock) s += pp[i];
+  const double t = block_sum<double>(s, red);
+  __shared__ double s_tot;
+  if (threadIdx.x == 0) s_tot = t;
+  __syncthreads();
+  const double r = s_tot;
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ int axis_len(int i, int g) { return 1 + (i > 0) + (i < g - 1); }
+
+// Assemble row `id` of problem P (element scratch K/F), apply both Dirichlet
+// passes, store it into the SELL slices; returns b[id].
+__device__ double assemble_row(const BatchProb& P, int id, const double* K, const double* F,
+                               const int* soff, int* scols, double* svals) {
+  const int Nx = P.Nx, Ny = P.Ny, Nz = P.Nz;
+  const int gx = Nx + 1, gy = Ny + 1;
+  const int64_t ne = P.ne;
+  const int ix = id % gx, iy = (id / gx) % gy, iz = id / (gx * gy);
+  const int lox = ix > 0 ? -1 : 0, hix = ix < Nx ? 1 : 0;
+  const int loy = iy > 0 ? -1 : 0, hiy = iy < Ny ? 1 : 0;
+  const int loz = iz > 0 ? -1 : 0, hiz = iz < Nz ? 1 : 0;
+  double acc[27];
+#pragma unroll
+  for (int s = 0; s < 27; ++s) acc[s] = 0.0;
+  double accb = 0.0;
+#pragma unroll
+  for (int dz = -1; dz <= 0; ++dz)
+#pragma unroll
+    for (int dy = -1; dy <= 0; ++dy)
+#pragma unroll
+      for (int dx = -1; dx <= 0; ++dx) {
+        const int ex = ix + dx, ey = iy + dy, ez = iz + dz;
+        if (ex < 0 || ex >= Nx || ey < 0 || ey >= Ny || ez < 0 || ez >= Nz) continue;
+        const int64_t e = ex + (int64_t)Nx * (ey + (int64_t)Ny * ez);
+        const int cx = -dx, cy = -dy, cz = -dz;
+        const int li = corner_of(cx, cy, cz);
+#pragma unroll
+        for (int lj = 0; lj < 8; ++lj) {
+          const int s = (corner_z(lj) - cz + 1) * 9 + (corner_y(lj) - cy + 1) * 3 +
+                        (corner_x(lj) - cx + 1);
+          acc[s] += K[(int64_t)packed(li, lj) * ne + e];
+        }
+        accb += F[(int64_t)li * ne + e];
+      }
+  // Dirichlet, pass 0 (value 0) then pass 1 (value 1): impose_dirichlet's
+  // row rule (SparseMatrix_functions.hpp:431-480) on this row.
+  const bool in0 = ix == 0 || iy == 0 || iz == 0 || iy == gy - 1 || iz == Nz;
+  const bool in1 = ix == gx - 1;
+#pragma unroll
+  for (int pass = 0; pass < 2; ++pass) {
+    const double value = pass == 0 ? 0.0 : 1.0;
+    const bool row_bc = pass == 0 ? in0 : in1;
+    double sum = 0.0;
+#pragma unroll
+    for (int sz = -1; sz <= 1; ++sz)
+#pragma unroll
+      for (int sy = -1; sy <= 1; ++sy)
+#pragma unroll
+        for (int sx = -1; sx <= 1; ++sx) {
+          const int s = (sz + 1) * 9 + (sy + 1) * 3 + (sx + 1);
+          const bool in = sx >= lox && sx <= hix && sy >= loy && sy <= hiy && sz >= loz && sz <= hiz;
+          if (!in) continue;
+          if (row_bc) {
+            acc[s] = (sx == 0 && sy == 0 && sz == 0) ? 1.0 : 0.0;
+          } else {
+            const int jx = ix + sx, jy = iy + sy, jz = iz + sz;
+            const bool col_bc = pass == 0
+                                    ? (jx == 0 || jy == 0 || jz == 0 || jy == gy - 1 || jz == Nz)
+                                    : (jx == gx - 1);
+            if (col_bc) {
+              sum += acc[s];
+              acc[s] = 0.0;
+            }
+          }
+        }
+    if (row_bc)
+      accb = value;
+    else
+      accb -= sum * value;
+  }
+  // Store in ascending column order into the row's SELL slots; pad the rest.
+  const int slice = id / kWave, lane = id % kWave;
+  const int o = soff[slice], w = soff[slice + 1] - o;
+  const int64_t base = P.slot0 + (int64_t)o * kWave + lane;
+  int pos = 0;
+#pragma unroll
+  for (int sz = -1; sz <= 1; ++sz)
+#pragma unroll
+    for (int sy = -1; sy <= 1; ++sy)
+#pragma unroll
+      for (int sx = -1; sx <= 1; ++sx) {
+        const int s = (sz + 1) * 9 + (sy + 1) * 3 + (sx + 1);
+        const bool in = sx >= lox && sx <= hix && sy >= loy && sy <= hiy && sz >= loz && sz <= hiz;
+        if (!in) continue;
+        svals[base + (int64_t)pos * kWave] = acc[s];
+        scols[base + (int64_t)pos * kWave] = (ix + sx) + gx * ((iy + sy) + gy * (iz + sz));
+        ++pos;
+      }
+  for (; pos < w; ++pos) {
+    svals[base + (int64_t)pos * kWave] = 0.0;
+    scols[base + (int64_t)pos * kWave] = id;
+  }
+  return accb;
+}
+
+// Padding of a row slot past n in the problem's last slice: value 0 and a
+// valid column, so the SpMV's unconditional loads and gathers stay in bounds.
+__device__ void pad_row(const BatchProb& P, int id, const int* soff, int* scols, double* svals) {
+  const int slice = id / kWave, lane = id % kWave;
+  const int o = soff[slice], w = soff[slice + 1] - o;
+  const int64_t base = P.slot0 + (int64_t)o * kWave + lane;
+  for (int j = 0; j < w; ++j) {
+    svals[base + (int64_t)j * kWave] = 0.0;
+    scols[base + (int64_t)j * kWave] = P.n - 1;
+  }
+}
+
+// (A p)[row] for row < n of problem P from its SELL slices, summed over the
+// row's entries in stored column order (the row length is the box stencil's).
+__device__ __forceinline__ double sell_row_sum(const BatchProb& P, int row, const int* soff,
+                                               const int* scols, const double* svals,
+                                               const double* pv) {
+  const int slice = row / kWave, lane = row % kWave;
+  const int o = soff[slice], w = soff[slice + 1] - o;
+  const int gx = P.Nx + 1, gy = P.Ny + 1;
+  const int rr = row < P.n ? row : P.n - 1;
+  const int ix = rr % gx, iy = (rr / gx) % gy, iz = rr / (gx * gy);
+  const int len = row < P.n ? axis_len(ix, gx) * axis_len(iy, gy) * axis_len(iz, P.Nz + 1) : 0;
+  double sum = 0.0;
+  if (w > 0) {
+    const int64_t base = P.slot0 + (int64_t)o * kWave + lane;
+    int c[kSellMaxWidth];
+    double v[kSellMaxWidth], xv[kSellMaxWidth];
+#pragma unroll
+    for (int j = 0; j < kSellMaxWidth; ++j) {
+      const int64_t at = base + (int64_t)(j < w ? j : w - 1) * kWave;
+      c[j] = scols[at];
+      v[j] = svals[at];
+    }
+#pragma unroll
+    for (int j = 0; j < kSellMaxWidth; ++j) xv[j] = pv[c[j]];
+#pragma unroll
+    for (int j = 0; j < kSellMaxWidth; ++j)
+      if (j < len) sum += v[j] * xv[j];
+  }
+  return sum;
+}
+
+__global__ __launch_bounds__(kBlock) void batch_kernel(
+    const BatchProb* __restrict__ probs, const int* __restrict__ blk2prob, HexTables tab,
+    const int* __restrict__ soff_all, int* scols, double* svals, double* KF, double* b,
+    double* x, double* r, double* p, double* Ap, double* partials, BatchState* states,
+    int* abort_flag, int max_iter, double rel_tol, uint64_t limit_ticks) {
+  __shared__ double red[4];
+  const int pi = blk2prob[blockIdx.x];
+  const BatchProb P = probs[pi];
+  BatchState* st = states + pi;
+  const int i = (int)blockIdx.x - P.b0, nb = P.nb, t = (int)threadIdx.x;
+  const int64_t ne = P.ne;
+  double* K = KF + P.elem0;
+  double* F = K + 36 * ne;
+  const int* soff = soff_all + P.soff0;
+  double* bp = b + P.row0;
+  double* xp = x + P.row0;
+  double* rp = r + P.row0;
+  double* pp = p + P.row0;
+  double* App = Ap + P.row0;
+  double* part = partials + P.part0;
+  const int ntiles = (int)cdiv((int64_t)P.nslices * kWave, kBlock);
+
+  // 1. element matrices and source vectors (as element_kernel, k = 1)
+  {
+    const double hx = 1.0 / P.Nx, hy = 1.0 / P.Ny, hz = 1.0 / P.Nz;
+    for (int64_t e = (int64_t)i * kBlock + t; e < ne; e += (int64_t)nb * kBlock) {
+      const int ex = (int)(e % P.Nx);
+      const int ey = (int)((e / P.Nx) % P.Ny);
+      const int ez = (int)(e / ((int64_t)P.Nx * P.Ny));
+      const double x0 = 1.0 * ex / P.Nx;
+      const double y0 = 1.0 * ey / P.Ny;
+      const double z0 = (1.0 * ez) / P.Nz;
+      const double x1 = x0 + hx, y1 = y0 + hy, z1 = z0 + hz;
+      const double c[24] = {x0, y0, z0, x1, y0, z0, x1, y1, z0, x0, y1, z0,
+                            x0, y0, z1, x1, y0, z1, x1, y1, z1, x0, y1, z1};
+      double Ke[36], Fe[8];
+      hex8_element(c, tab, 1.0, Ke, Fe);
+#pragma unroll
+      for (int q = 0; q < 36; ++q) K[(int64_t)q * ne + e] = Ke[q];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) F[(int64_t)q * ne + e] = Fe[q];
+    }
+  }
+  if (!prob_barrier(st, nb, abort_flag, limit_ticks)) return;
+
+  // 2. rows: gather, Dirichlet, SELL store; x = 0, r = b
+  double acc = 0.0;
+  for (int tile = i; tile < ntiles; tile += nb) {
+    const int row = tile * kBlock + t;
+    if (row < P.n) {
+      const double bv = assemble_row(P, row, K, F, soff, scols, svals);
+      bp[row] = bv;
+      xp[row] = 0.0;
+      rp[row] = bv;
+      acc += bv * bv;
+    } else if (row < P.nslices * kWave) {
+      pad_row(P, row, soff, scols, svals);
+    }
+  }
+  {
+    const double v0 = block_sum<double>(acc, red);
+    if (t == 0) part[i] = v0;
+  }
+  if (!prob_barrier(st, nb, abort_flag, limit_ticks)) return;
+  double rtrans = sum_partials_all(part, nb, red);
+  int buf = 1;
+  const double bnorm = sqrt(rtrans);
+  const double tol = rel_tol * bnorm;
+  double normr = bnorm, oldrtrans = 0.0;
+  int status = 0, k = 1;
+
+  // 3. CG
+  for (; k <= max_iter && normr > tol; ++k) {
+    const double beta = k == 1 ? 0.0 : rtrans / oldrtrans;
+    for (int tile = i; tile < ntiles; tile += nb) {
+      const int row = tile * kBlock + t;
+      if (row < P.n) pp[row] = k == 1 ? 1.0 * rp[row] + 0.0 * rp[row] : 1.0 * rp[row] + beta * pp[row];
+    }
+    if (!prob_barrier(st, nb, abort_flag, limit_ticks)) return;
+    acc = 0.0;
+    for (int tile = i; tile < ntiles; tile += nb) {
+      const int row = tile * kBlock + t;
+      if (row / kWave < P.nslices) {
+        const double sum = sell_row_sum(P, row, soff, scols, svals, pp);
+        if (row < P.n) {
+          App[row] = sum;
+          acc += sum * pp[row];
+        }
+      }
+    }
+    {
+      const double v0 = block_sum<double>(acc, red);
+      if (t == 0) part[buf * nb + i] = v0;
+    }
+    if (!prob_barrier(st, nb, abort_flag, limit_ticks)) return;
+    const double pAp = sum_partials_all(part + buf * nb, nb, red);
+    buf ^= 1;
+    if (!(pAp > 0.0)) {  // non-positive curvature: the operator is not SPD here
+      status = 1;
+      break;
+    }
+    const double alpha = rtrans / pAp;
+    acc = 0.0;
+    for (int tile = i; tile < ntiles; tile += nb) {
+      const int row = tile * kBlock + t;
+      if (row < P.n) {
+        xp[row] = 1.0 * xp[row] + alpha * pp[row];
+        const double rv = 1.0 * rp[row] + (-alpha) * App[row];
+        rp[row] = rv;
+        acc += rv * rv;
+      }
+    }
+    {
+      const double v0 = block_sum<double>(acc, red);
+      if (t == 0) part[buf * nb + i] = v0;
+    }
+    if (!prob_barrier(st, nb, abort_flag, limit_ticks)) return;
+    oldrtrans = rtrans;
+    rtrans = sum_partials_all(part + buf * nb, nb, red);
+    buf ^= 1;
+    normr = sqrt(rtrans);
+  }
+  if (i == 0 && t == 0) {
+    st->iters = k - 1;
+    st->status = status;
+    st->normr = normr;
+    st->tol = tol;
+    st->bnorm = bnorm;
+  }
+}
+
+}  // namespace mfe
+
+using namespace mfe;
+
+struct mfe_batch {
+  mfe_ctx* ctx = nullptr;
+  int nprob = 0, grid = 0;
+  std::vector<BatchProb> probs;
+  std::vector<int> soff;  // all problems' slice offsets
+  BatchProb* d_probs = nullptr;
+  int* d_blk2prob = nullptr;
+  int* d_soff = nullptr;
+  int* d_scols = nullptr;
+  double* d_svals = nullptr;
+  double* d_KF = nullptr;
+  double *d_b = nullptr, *d_x = nullptr, *d_r = nullptr, *d_p = nullptr, *d_Ap = nullptr;
+  double* d_part = nullptr;
+  BatchState* d_states = nullptr;
+  int* d_abort = nullptr;
+  int64_t rows = 0, slots = 0;
+  void release() {
+    (void)hipFree(d_probs);
+    (void)hipFree(d_blk2prob);
+    (void)hipFree(d_soff);
+    (void)hipFree(d_scols);
+    (void)hipFree(d_svals);
+    (void)hipFree(d_KF);
+    (void)hipFree(d_b);
+    (void)hipFree(d_x);
+    (void)hipFree(d_r);
+    (void)hipFree(d_p);
+    (void)hipFree(d_Ap);
+    (void)hipFree(d_part);
+    (void)hipFree(d_states);
+    (void)hipFree(d_abort);
+  }
+};
+
+static int host_axis_len(int i, int g) { return 1 + (i > 0) + (i < g - 1); }
+
+extern "C" {
+
+int mfe_batch_create(mfe_ctx* ctx, int nprob, const int* dims, mfe_batch** out) {
+  if (!ctx || !out || nprob <= 0 || !dims) return MFE_BAD_ARG;
+  *out = nullptr;
+  for (int q = 0; q < 3 * nprob; ++q)
+    if (dims[q] < 1 || dims[q] > 1024)
+      return fail(ctx, MFE_BAD_ARG, "batch_create: box sides must be in [1, 1024] elements");
+  MFE_CHECK(ctx, hipSetDevice(ctx->device));
+  int per_cu = 0;
+  MFE_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel, kBlock, 0));
+  int coop = 0;
+  MFE_CHECK(ctx, hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, ctx->device));
+  const int grid = per_cu * ctx->cu_count;
+  if (!coop || grid < nprob)
+    return fail(ctx, MFE_UNSUPPORTED, "batch_create: needs a cooperative launch with at least one "
+                                      "resident workgroup per problem");
+  mfe_batch* B = new mfe_batch();
+  B->ctx = ctx;
+  B->nprob = nprob;
+  B->grid = grid;
+  // Problem layout and slice widths (closed form: a row's length is the
+  // product of its per-axis neighbour counts).
+  std::vector<double> work((size_t)nprob);
+  int64_t row0 = 0, soff0 = 0, slot0 = 0, elem0 = 0;
+  for (int q = 0; q < nprob; ++q) {
+    BatchProb P{};
+    P.Nx = dims[3 * q];
+    P.Ny = dims[3 * q + 1];
+    P.Nz = dims[3 * q + 2];
+    const int gx = P.Nx + 1, gy = P.Ny + 1, gz = P.Nz + 1;
+    P.n = gx * gy * gz;
+    P.ne = P.Nx * P.Ny * P.Nz;
+    P.nslices = (int)cdiv(P.n, kWave);
+    P.row0 = row0;
+    P.soff0 = soff0;
+    P.slot0 = slot0;
+    P.elem0 = elem0;
+    int acc = 0;
+    for (int s = 0; s < P.nslices; ++s) {
+      B->soff.push_back(acc);
+      int w = 0;
+      for (int l = 0; l < kWave; ++l) {
+        const int id = s * kWave + l;
+        if (id >= P.n) break;
+        const int ix = id % gx, iy = (id / gx) % gy, iz = id / (gx * gy);
+        w = std::max(w, host_axis_len(ix, gx) * host_axis_len(iy, gy) * host_axis_len(iz, gz));
+      }
+      acc += w;
+    }
+    B->soff.push_back(acc);
+    row0 += P.n;
+    soff0 += P.nslices + 1;
+    slot0 += (int64_t)acc * kWave;
+    elem0 += 44 * (int64_t)P.ne;
+    const double nnz = (3.0 * gx - 2) * (3.0 * gy - 2) * (3.0 * gz - 2);
+    work[(size_t)q] = nnz * (double)(P.Nx + P.Ny + P.Nz);
+    B->probs.push_back(P);
+  }
+  B->rows = row0;
+  B->slots = slot0;
+  // Workgroups per problem: one each, then the rest one at a time to the
+  // problem with the most work per workgroup (capped at its tile count).
+  std::vector<int> nb((size_t)nprob, 1);
+  int left = grid - nprob;
+  while (left > 0) {
+    int best = -1;
+    double bw = -1;
+    for (int q = 0; q < nprob; ++q) {
+      const int tiles = (int)cdiv((int64_t)B->probs[(size_t)q].nslices * kWave, kBlock);
+      if (nb[(size_t)q] >= tiles) continue;
+      const double w = work[(size_t)q] / nb[(size_t)q];
+      if (w > bw) {
+        bw = w;
+        best = q;
+      }
+    }
+    if (best < 0) break;
+    ++nb[(size_t)best];
+    --left;
+  }
+  std::vector<int> blk2prob;
+  int64_t part0 = 0;
+  for (int q = 0; q < nprob; ++q) {
+    BatchProb& P = B->probs[(size_t)q];
+    P.b0 = (int)blk2prob.size();
+    P.nb = nb[(size_t)q];
+    P.part0 = part0;
+    part0 += 2 * (int64_t)P.nb;
+    for (int k = 0; k < P.nb; ++k) blk2prob.push_back(q);
+  }
+  B->grid = (int)blk2prob.size();
+  hipStream_t s = ctx->stream;
+  auto fail_oom = [&](hipError_t e) {
+    B->release();
+    delete B;
+    return hip_fail(ctx, e, "batch_create: hipMalloc");
+  };
+  hipError_t e;
+#define BALLOC(ptr, bytes) \
+  if ((e = hipMalloc(&(ptr), (bytes))) != hipSuccess) return fail_oom(e)
+  BALLOC(B->d_probs, sizeof(BatchProb) * (size_t)nprob);
+  BALLOC(B->d_blk2prob, sizeof(int) * blk2prob.size());
+  BALLOC(B->d_soff, sizeof(int) * B->soff.size());
+  BALLOC(B->d_scols, sizeof(int) * (size_t)B->slots);
+  BALLOC(B->d_svals, sizeof(double) * (size_t)B->slots);
+  BALLOC(B->d_KF, sizeof(double) * (size_t)elem0);
+  BALLOC(B->d_b, sizeof(double) * (size_t)B->rows);
+  BALLOC(B->d_x, sizeof(double) * (size_t)B->rows);
+  BALLOC(B->d_r, sizeof(double) * (size_t)B->rows);
+  BALLOC(B->d_p, sizeof(double) * (size_t)B->rows);
+  BALLOC(B->d_Ap, sizeof(double) * (size_t)B->rows);
+  BALLOC(B->d_part, sizeof(double) * (size_t)part0);
+  BALLOC(B->d_states, sizeof(BatchState) * (size_t)nprob);
+  BALLOC(B->d_abort, sizeof(int));
+#undef BALLOC
+  MFE_CHECK(ctx, hipMemcpyAsync(B->d_probs, B->probs.data(), sizeof(BatchProb) * (size_t)nprob,
+                                hipMemcpyHostToDevice, s));
+  MFE_CHECK(ctx, hipMemcpyAsync(B->d_blk2prob, blk2prob.data(), sizeof(int) * blk2prob.size(),
+                                hipMemcpyHostToDevice, s));
+  MFE_CHECK(ctx, hipMemcpyAsync(B->d_soff, B->soff.data(), sizeof(int) * B->soff.size(),
+                                hipMemcpyHostToDevice, s));
+  MFE_CHECK(ctx, hipStreamSynchronize(s));
+  *out = B;
+  return MFE_OK;
+}
+
+void mfe_batch_destroy(mfe_batch* B) {
+  if (!B) return;
+  (void)hipStreamSynchronize(B->ctx->stream);
+  B->release();
+  delete B;
+}
+
+int mfe_batch_info(const mfe_batch* B, int p, int64_t info[6]) {
+  if (!B || !info || p < -1 || p >= B->nprob) return MFE_BAD_ARG;
+  if (p == -1) {
+    info[0] = B->nprob;
+    info[1] = B->grid;
+    info[2] = B->rows;
+    info[3] = B->slots;
+    int64_t nnz = 0;
+    for (const BatchProb& P : B->probs)
+      nnz += (int64_t)(3 * P.Nx + 1) * (3 * P.Ny + 1) * (3 * P.Nz + 1);
+    info[4] = nnz;
+    info[5] = 0;
+    return MFE_OK;
+  }
+  const BatchProb& P = B->probs[(size_t)p];
+  info[0] = P.n;
+  info[1] = (int64_t)(3 * P.Nx + 1) * (3 * P.Ny + 1) * (3 * P.Nz + 1);
+  info[2] = P.ne;
+  info[3] = P.nb;
+  info[4] = P.b0;
+  info[5] = (int64_t)B->soff[(size_t)(P.soff0 + P.nslices)] * kWave;
+  return MFE_OK;
+}
+
+int mfe_batch_solve(mfe_ctx* ctx, mfe_batch* B, int max_iter, double rel_tol, int* iters,
+                    double* rel_residual, double* seconds) {
+  if (!ctx || !B || B->ctx != ctx || max_iter < 0 || !(rel_tol >= 0)) return MFE_BAD_ARG;
+  MFE_CHECK(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  MFE_CHECK(ctx, hipMemsetAsync(B->d_states, 0, sizeof(BatchState) * (size_t)B->nprob, s));
+  MFE_CHECK(ctx, hipMemsetAsync(B->d_abort, 0, sizeof(int), s));
+  int rate_khz = 0;
+  MFE_CHECK(ctx, hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
+  uint64_t limit = (uint64_t)(kBatchWaitSeconds * 1e3 * (double)(rate_khz > 0 ? rate_khz : 100000));
+  HexTables tab;
+  host_tables(tab);
+  void* args[] = {&B->d_probs, &B->d_blk2prob, &tab,        &B->d_soff,   &B->d_scols,
+                  &B->d_svals, &B->d_KF,       &B->d_b,     &B->d_x,      &B->d_r,
+                  &B->d_p,     &B->d_Ap,       &B->d_part,  &B->d_states, &B->d_abort,
+                  &max_iter,   &rel_tol,       &limit};
+  MFE_CHECK(ctx, hipEventRecord(ctx->t_start, s));
+  MFE_CHECK(ctx, hipLaunchCooperativeKernel((const void*)batch_kernel, dim3((unsigned)B->grid),
+                                            dim3(kBlock), args, 0, s));
+  MFE_CHECK(ctx, hipEventRecord(ctx->t_stop, s));
+  MFE_CHECK(ctx, hipEventSynchronize(ctx->t_stop));
+  float ms = 0;
+  MFE_CHECK(ctx, hipEventElapsedTime(&ms, ctx->t_start, ctx->t_stop));
+  if (seconds) *seconds = 1e-3 * ms;
+  int aborted = 0;
+  std::vector<BatchState> hs((size_t)B->nprob);
+  MFE_CHECK(ctx, hipMemcpy(&aborted, B->d_abort, sizeof(int), hipMemcpyDeviceToHost));
+  MFE_CHECK(ctx, hipMemcpy(hs.data(), B->d_states, sizeof(BatchState) * hs.size(),
+                           hipMemcpyDeviceToHost));
+  if (aborted)
+    return fail(ctx, MFE_HIP_ERROR, "batch_solve: a problem barrier waited past its limit; "
+                                    "the launch was abandoned");
+  int status = MFE_OK;
+  for (int q = 0; q < B->nprob; ++q) {
+    if (iters) iters[q] = hs[(size_t)q].iters;
+    if (rel_residual)
+      rel_residual[q] = hs[(size_t)q].bnorm > 0 ? hs[(size_t)q].normr / hs[(size_t)q].bnorm
+                                                : hs[(size_t)q].normr;
+    if (hs[(size_t)q].status == 1) status = MFE_BREAKDOWN;
+  }
+  if (status == MFE_BREAKDOWN) fail(ctx, MFE_BREAKDOWN, "batch_solve: CG breakdown in a problem");
+  return status;
+}
+
+int mfe_batch_download(mfe_ctx* ctx, const mfe_batch* B, int p, int* row_offsets, int* cols,
+                       double* vals, double* b, double* x) {
+  if (!ctx || !B || p < 0 || p >= B->nprob) return MFE_BAD_ARG;
+  MFE_CHECK(ctx, hipSetDevice(ctx->device));
+  const BatchProb& P = B->probs[(size_t)p];
+  if (b) MFE_CHECK(ctx, hipMemcpy(b, B->d_b + P.row0, sizeof(double) * (size_t)P.n, hipMemcpyDeviceToHost));
+  if (x) MFE_CHECK(ctx, hipMemcpy(x, B->d_x + P.row0, sizeof(double) * (size_t)P.n, hipMemcpyDeviceToHost));
+  if (!row_offsets && !cols && !vals) return MFE_OK;
+  const int* so = B->soff.data() + P.soff0;
+  const int64_t nslot = (int64_t)so[P.nslices] * kWave;
+  std::vector<int> sc((size_t)nslot);
+  std::vector<double> sv((size_t)nslot);
+  MFE_CHECK(ctx, hipMemcpy(sc.data(), B->d_scols + P.slot0, sizeof(int) * (size_t)nslot,
+                           hipMemcpyDeviceToHost));
+  MFE_CHECK(ctx, hipMemcpy(sv.data(), B->d_svals + P.slot0, sizeof(double) * (size_t)nslot,
+                           hipMemcpyDeviceToHost));
+  const int gx = P.Nx + 1, gy = P.Ny + 1, gz = P.Nz + 1;
+  int64_t pos = 0;
+  for (int id = 0; id < P.n; ++id) {
+    const int ix = id % gx, iy = (id / gx) % gy, iz = id / (gx * gy);
+    const int len = host_axis_len(ix, gx) * host_axis_len(iy, gy) * host_axis_len(iz, gz);
+    if (row_offsets) row_offsets[id] = (int)pos;
+    const int s = id / kWave, l = id % kWave;
+    for (int j = 0; j < len; ++j, ++pos) {
+      const int64_t at = ((int64_t)so[s] + j) * kWave + l;
+      if (cols) cols[pos] = sc[(size_t)at];
+      if (vals) vals[pos] = sv[(size_t)at];
+    }
+  }
+  if (row_offsets) row_offsets[P.n] = (int)pos;
+  return MFE_OK;
+}
+
+}  // extern "C"

@@ -83,7 +83,7 @@ int hip_fail(mfe_ctx* ctx, hipError_t e, const char* what);
 // Recompute A->max_block_nnz on the device (synchronises the stream).
 int plan_spmv(mfe_ctx* ctx, mfe_csr* A);
 
-inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+__host__ __device__ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Derived copies (FP32 values, SELL plan) are stale after these.
 inline void csr_values_changed(mfe_csr* A) {

@@ -1,0 +1,143 @@
+// Hex8 element arithmetic shared by the assembly kernels and the batched
+// solver: Gauss tables, corner layout, packed-triangle index and the element
+// diffusion matrix / source vector with the reference's operation order.
+#pragma once
+
+#include "mfe_common.hpp"
+
+namespace mfe {
+
+// Gauss-point tables, computed on the host with the reference's expressions
+// (gauss_pts.hpp:37-62, Hex8.hpp:43-113, Hex8_ElemData.hpp:37-61) and passed
+// by value (read through the scalar cache by every lane).
+struct HexTables {
+  double gv[192];  // grad_vals[gp][node][dim], gp in (ig,jg,kg) order
+  double psi[64];  // shape_fns[gp][node]
+  double gw[2];    // Gauss weights
+};
+
+inline void host_gauss_pts(double pts[2], double wts[2]) {
+  const double x2 = 0.577350269;  // truncated 1/sqrt(3), gauss_pts.hpp:41
+  pts[0] = -x2;
+  wts[0] = 1.0;
+  pts[1] = x2;
+  wts[1] = 1.0;
+}
+
+inline void host_tables(HexTables& t) {
+  double gp[2], gw[2];
+  host_gauss_pts(gp, gw);
+  t.gw[0] = gw[0];
+  t.gw[1] = gw[1];
+  int q = 0;
+  for (int ig = 0; ig < 2; ++ig)
+    for (int jg = 0; jg < 2; ++jg)
+      for (int kg = 0; kg < 2; ++kg, ++q) {
+        const double x0 = gp[ig], x1 = gp[jg], x2 = gp[kg];
+        const double u = 1.0 - x0, v = 1.0 - x1, w = 1.0 - x2;
+        const double up1 = 1.0 + x0, vp1 = 1.0 + x1, wp1 = 1.0 + x2;
+        double* g = t.gv + 24 * q;
+        g[0] = -0.125 * v * w;      g[1] = -0.125 * u * w;      g[2] = -0.125 * u * v;
+        g[3] = 0.125 * v * w;       g[4] = -0.125 * up1 * w;    g[5] = -0.125 * up1 * v;
+        g[6] = 0.125 * vp1 * w;     g[7] = 0.125 * up1 * w;     g[8] = -0.125 * up1 * vp1;
+        g[9] = -0.125 * vp1 * w;    g[10] = 0.125 * u * w;      g[11] = -0.125 * u * vp1;
+        g[12] = -0.125 * v * wp1;   g[13] = -0.125 * u * wp1;   g[14] = 0.125 * u * v;
+        g[15] = 0.125 * v * wp1;    g[16] = -0.125 * up1 * wp1; g[17] = 0.125 * up1 * v;
+        g[18] = 0.125 * vp1 * wp1;  g[19] = 0.125 * up1 * wp1;  g[20] = 0.125 * up1 * vp1;
+        g[21] = -0.125 * vp1 * wp1; g[22] = 0.125 * u * wp1;    g[23] = 0.125 * u * vp1;
+        double* s = t.psi + 8 * q;
+        s[0] = 0.125 * u * v * w;
+        s[1] = 0.125 * up1 * v * w;
+        s[2] = 0.125 * up1 * vp1 * w;
+        s[3] = 0.125 * u * vp1 * w;
+        s[4] = 0.125 * u * v * wp1;
+        s[5] = 0.125 * up1 * v * wp1;
+        s[6] = 0.125 * up1 * vp1 * wp1;
+        s[7] = 0.125 * u * vp1 * wp1;
+      }
+}
+
+// Exodus hex8 corner layout (Hex8_box_utils.hpp:41-57).
+__host__ __device__ constexpr int corner_x(int c) { return (c == 1 || c == 2 || c == 5 || c == 6); }
+__host__ __device__ constexpr int corner_y(int c) { return (c == 2 || c == 3 || c == 6 || c == 7); }
+__host__ __device__ constexpr int corner_z(int c) { return c >= 4; }
+__host__ __device__ constexpr int corner_of(int x, int y, int z) {
+  return z * 4 + (y ? (x ? 2 : 3) : (x ? 1 : 0));
+}
+// Packed upper triangle index of (m, n), row-major m <= n (Hex8.hpp:324-341).
+__host__ __device__ constexpr int packed(int a, int b) {
+  return a <= b ? a * 8 - a * (a - 1) / 2 + (b - a) : b * 8 - b * (b - 1) / 2 + (a - b);
+}
+
+// Hex8::diffusionMatrix_symm (Hex8.hpp:259-361) and Hex8::sourceVector
+// (Hex8.hpp:363-410) for one element; J, detJ are evaluated once per Gauss
+// point and shared (the reference evaluates the same expressions twice).
+__device__ __forceinline__ void hex8_element(const double c[24], const HexTables& tab, double k,
+                                             double K[36], double F[8]) {
+#pragma unroll
+  for (int i = 0; i < 36; ++i) K[i] = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) F[i] = 0;
+  int q = 0;
+#pragma unroll
+  for (int ig = 0; ig < 2; ++ig) {
+    const double wi = tab.gw[ig];
+#pragma unroll
+    for (int jg = 0; jg < 2; ++jg) {
+      const double wj = tab.gw[jg];
+      const double wi_wj = wi * wj;
+#pragma unroll
+      for (int kg = 0; kg < 2; ++kg, ++q) {
+        const double wk = tab.gw[kg];
+        const double wi_wj_wk = wi_wj * wk;
+        const double* g = tab.gv + 24 * q;
+        double J00 = 0, J01 = 0, J02 = 0, J10 = 0, J11 = 0, J12 = 0, J20 = 0, J21 = 0, J22 = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const double g0 = g[3 * i], g1 = g[3 * i + 1], g2 = g[3 * i + 2];
+          const double x = c[3 * i], y = c[3 * i + 1], z = c[3 * i + 2];
+          J00 += g0 * x; J01 += g0 * y; J02 += g0 * z;
+          J10 += g1 * x; J11 += g1 * y; J12 += g1 * z;
+          J20 += g2 * x; J21 += g2 * y; J22 += g2 * z;
+        }
+        const double term0 = J22 * J11 - J21 * J12;
+        const double term1 = J22 * J01 - J21 * J02;
+        const double term2 = J12 * J01 - J11 * J02;
+        const double detJ = J00 * term0 - J10 * term1 + J20 * term2;
+        const double inv_detJ = 1.0 / detJ;
+        const double i0 = term0 * inv_detJ;
+        const double i1 = -term1 * inv_detJ;
+        const double i2 = term2 * inv_detJ;
+        const double i3 = -(J22 * J10 - J20 * J12) * inv_detJ;
+        const double i4 = (J22 * J00 - J20 * J02) * inv_detJ;
+        const double i5 = -(J12 * J00 - J10 * J02) * inv_detJ;
+        const double i6 = (J21 * J10 - J20 * J11) * inv_detJ;
+        const double i7 = -(J21 * J00 - J20 * J01) * inv_detJ;
+        const double i8 = (J11 * J00 - J10 * J01) * inv_detJ;
+        const double kdw = k * detJ * wi_wj_wk;
+        double dx[8], dy[8], dz[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const double g0 = g[3 * i], g1 = g[3 * i + 1], g2 = g[3 * i + 2];
+          dx[i] = g0 * i0 + g1 * i1 + g2 * i2;
+          dy[i] = g0 * i3 + g1 * i4 + g2 * i5;
+          dz[i] = g0 * i6 + g1 * i7 + g2 * i8;
+        }
+        int off = 0;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          K[off++] += kdw * ((dx[m] * dx[m]) + (dy[m] * dy[m]) + (dz[m] * dz[m]));
+#pragma unroll
+          for (int n = m + 1; n < 8; ++n)
+            K[off++] += kdw * ((dx[m] * dx[n]) + (dy[m] * dy[n]) + (dz[m] * dz[n]));
+        }
+        // sourceVector: term = Q*detJ*wi*wj*wk with Q = 1.0 (Hex8.hpp:379,402).
+        const double term = 1.0 * detJ * wi * wj * wk;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) F[i] += tab.psi[8 * q + i] * term;
+      }
+    }
+  }
+}
+
+}  // namespace mfe

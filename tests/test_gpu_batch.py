@@ -1,0 +1,99 @@
+"""GPU batched boxes: many independent miniFE problems assembled and solved by
+FP64 CG in one cooperative persistent launch.
+
+Checks: every problem's assembled operator (structure, values, right-hand
+side) is bit-identical to the oracle's single-box assembly; the solutions
+reach the requested residual when it is recomputed on the CPU from the
+downloaded operator; iteration counts match the oracle's serial CG to within
+the rounding of the tree-reduced dots; the 64-box workload (sides uniform on
+[16, 96], seed 0) converges in every box to 1e-10.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import bits_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def mfe():
+    import minife_amd
+    minife_amd.lib()
+    return minife_amd
+
+
+def check_problem(mfe, oracle_mod, B, p, dims, rel_tol, exact=True):
+    got = B.download(p)
+    P = oracle_mod.build_problem(*dims)
+    A = P["A"]
+    assert np.array_equal(got["row_offsets"], A.row_offsets), dims
+    assert np.array_equal(got["cols"], A.cols), dims
+    if exact:
+        assert bits_equal(got["vals"], A.vals), dims
+        assert bits_equal(got["b"], P["b"]), dims
+    b = P["b"]
+    res = b - oracle_mod.matvec(A, got["x"])
+    true_rel = float(np.linalg.norm(res) / np.linalg.norm(b))
+    assert true_rel <= 4 * rel_tol, (dims, true_rel)
+    return P, got, true_rel
+
+
+def test_small_batch_matches_oracle(mfe, oracle_mod):
+    dims = [(1, 1, 1), (2, 3, 4), (17, 3, 5), (10, 10, 10), (3, 4, 5), (5, 1, 2), (1, 9, 1)]
+    say(f"batch small {dims}")
+    B = mfe.Batch(dims)
+    say(f"  info {B.info()}")
+    out = B.solve(max_iter=2000, rel_tol=1e-10)
+    assert out["status"] == mfe.MFE_OK
+    say(f"  iters {out['iters'].tolist()} rel {out['rel_residual'].max():.2e}")
+    for p, d in enumerate(dims):
+        P, got, true_rel = check_problem(mfe, oracle_mod, B, p, d, 1e-10)
+        bn = float(np.sqrt(oracle_mod.dot(P["b"], P["b"])))
+        r = oracle_mod.cg_solve(P["A"], P["b"], max_iter=2000, tol=1e-10 * bn)
+        assert abs(int(out["iters"][p]) - r["num_iters"]) <= 2, (d, out["iters"][p], r["num_iters"])
+        xs = r["x"]
+        assert np.max(np.abs(got["x"] - xs)) <= 1e-8 * max(1.0, np.max(np.abs(xs)))
+    # a second solve of the same batch gives the same bits (deterministic)
+    x0 = B.download(3, matrix=False)["x"]
+    B.solve(max_iter=2000, rel_tol=1e-10)
+    assert bits_equal(B.download(3, matrix=False)["x"], x0)
+    # the 10^3 box verifies against the analytic solution like the driver
+    assert oracle_mod.verify_solution(10, 10, 10, x0)[0] == 0
+
+
+def test_batch_max_iter_and_zero_iterations(mfe, oracle_mod):
+    say("batch max_iter")
+    B = mfe.Batch([(6, 5, 4), (3, 3, 3)])
+    out = B.solve(max_iter=3, rel_tol=1e-12)
+    assert out["iters"].tolist() == [3, 3]
+    out = B.solve(max_iter=0, rel_tol=1e-12)
+    assert out["iters"].tolist() == [0, 0]
+    assert np.all(B.download(0, matrix=False)["x"] == 0.0)
+
+
+def test_batch_64_boxes(mfe, oracle_mod):
+    from minife_amd.workloads import batch_boxes
+    dims = batch_boxes(0)
+    say(f"batch 64: sides {dims.min()}..{dims.max()}")
+    B = mfe.Batch(dims)
+    inf = B.info()
+    say(f"  info {inf}")
+    B.solve(max_iter=5000, rel_tol=1e-10)  # warm
+    out = B.solve(max_iter=5000, rel_tol=1e-10)
+    assert out["status"] == mfe.MFE_OK
+    assert np.all(out["rel_residual"] <= 1e-10)
+    say(f"  {out['seconds']:.3f} s, iters {out['iters'].min()}..{out['iters'].max()}, "
+        f"sum {int(out['iters'].sum())}")
+    vol = dims.prod(axis=1)
+    picks = {int(np.argmin(vol)), int(np.argmax(vol)), 7}
+    for p in sorted(picks):
+        _, _, tr = check_problem(mfe, oracle_mod, B, p, tuple(int(v) for v in dims[p]), 1e-10)
+        say(f"  problem {p} {tuple(dims[p])}: bit-exact operator, true rel residual {tr:.2e}")
