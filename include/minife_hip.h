@@ -245,6 +245,27 @@ int mfe_batch_solve(mfe_ctx* ctx, mfe_batch* batch, int max_iter, double rel_tol
 int mfe_batch_download(mfe_ctx* ctx, const mfe_batch* batch, int p, int* row_offsets, int* cols,
                        double* vals, double* b, double* x);
 
+/* ---- many right-hand sides on one matrix ----------------------------------- */
+/* (no reference counterpart: the reference solves one right-hand side.)
+ * Block vectors are row-major n x nrhs mfe_vecs of n*nrhs values; nrhs is 4,
+ * 8 or 16.  SpMM tiles: `tile` < 0 picks the autotuned tile for A and nrhs
+ * (tuned on first use); tile >= 0 forces one (for comparison).  Column k of
+ * Y = A X is bit-identical to mfe_matvec's A * X[:, k] for every tile. */
+int mfe_spmm(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* X, mfe_vec* Y, int nrhs, int tile);
+/* Autotuner: times every candidate tile on A (3 launches each, HIP events),
+ * keeps the fastest for A and nrhs; tile, ms[ntiles] and ntiles nullable. */
+int mfe_spmm_tune(mfe_ctx* ctx, const mfe_csr* A, int nrhs, int* tile, double* ms, int* ntiles);
+/* Shape of a tile: columns per lane, non-zeros per unrolled chunk, rows per
+ * 256-thread workgroup. */
+int mfe_spmm_tile_info(int nrhs, int tile, int* cols_per_lane, int* unroll, int* rows_per_block);
+/* nrhs simultaneous FP64 CG solves A X[:, k] = B[:, k] from the given X,
+ * sharing one SpMM per iteration; column k stops once ||r_k|| <= rel_tol *
+ * ||B[:, k]|| (cg_solve.hpp's loop test per column) or after max_iter.
+ * iters[nrhs], rel_residual[nrhs] (recursive), seconds: nullable. */
+int mfe_cg_solve_multi(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* B, mfe_vec* X, int nrhs,
+                       int max_iter, double rel_tol, int* iters, double* rel_residual,
+                       double* seconds);
+
 /* ---- verification (host) -------------------------------------------------- */
 /* soln(x, y, z, max_p, max_q), analytic_soln.hpp:80-112. */
 double mfe_soln(double x, double y, double z, int max_p, int max_q);

@@ -49,7 +49,8 @@ EXPORTS = [
     "mfe_matvec", "mfe_matvec_and_dot", "mfe_dot", "mfe_dot_replay", "mfe_waxpby",
     "mfe_cg_solve", "mfe_cg_solve_mixed", "mfe_matvec_fp32", "mfe_time_matvec", "mfe_soln",
     "mfe_verify_solution", "mfe_batch_create", "mfe_batch_destroy", "mfe_batch_info",
-    "mfe_batch_solve", "mfe_batch_download",
+    "mfe_batch_solve", "mfe_batch_download", "mfe_spmm", "mfe_spmm_tune", "mfe_spmm_tile_info",
+    "mfe_cg_solve_multi",
 ]
 
 
@@ -145,6 +146,14 @@ def lib() -> ctypes.CDLL:
         "mfe_time_matvec": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
                              ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "mfe_soln": ([ctypes.c_double] * 3 + [ctypes.c_int, ctypes.c_int], ctypes.c_double),
+        "mfe_spmm": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+        "mfe_spmm_tune": ([_vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp,
+                           ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "mfe_spmm_tile_info": ([ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+                               ctypes.c_int),
+        "mfe_cg_solve_multi": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                _vp, _vp, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "mfe_batch_create": ([_vp, ctypes.c_int, _vp, ctypes.POINTER(_vp)], ctypes.c_int),
         "mfe_batch_destroy": ([_vp], None),
         "mfe_batch_info": ([_vp, ctypes.c_int, _vp], ctypes.c_int),
@@ -537,6 +546,46 @@ def element_gradient(mesh: simple_mesh_description, lam: Vector, utilde: Vector,
     """grad[e] = -lambda_e^T K_e(1) utilde_e (the adjoint element gradient)."""
     lam.ctx.check(lib().mfe_element_gradient(lam.ctx.h, _box6(mesh.global_box), lam.h, utilde.h,
                                              grad.h), "element_gradient")
+
+
+def spmm(A: CSRMatrix, X: Vector, Y: Vector, nrhs: int, tile: int = -1) -> None:
+    """Y = A X for row-major n x nrhs block vectors (nrhs 4, 8 or 16); tile -1
+    uses the autotuned tile."""
+    A.ctx.check(lib().mfe_spmm(A.ctx.h, A.h, X.h, Y.h, int(nrhs), int(tile)), "spmm")
+
+
+def spmm_tune(A: CSRMatrix, nrhs: int) -> dict:
+    """Run the SpMM autotuner on A: {'tile': chosen id, 'ms': [...], 'tiles': [...]}."""
+    ms = np.zeros(16, np.float64)
+    tile, nt = ctypes.c_int(0), ctypes.c_int(0)
+    A.ctx.check(lib().mfe_spmm_tune(A.ctx.h, A.h, int(nrhs), ctypes.byref(tile), _ptr(ms),
+                                    ctypes.byref(nt)), "spmm_tune")
+    tiles = []
+    for i in range(nt.value):
+        c, u, r = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        lib().mfe_spmm_tile_info(int(nrhs), i, ctypes.byref(c), ctypes.byref(u), ctypes.byref(r))
+        tiles.append({"cols_per_lane": c.value, "unroll": u.value, "rows_per_block": r.value})
+    return {"tile": tile.value, "ms": ms[:nt.value].tolist(), "tiles": tiles}
+
+
+@dataclass
+class MultiResult:
+    iters: np.ndarray
+    rel_residual: np.ndarray
+    seconds: float
+
+
+def cg_solve_multi(A: CSRMatrix, B: Vector, X: Vector, nrhs: int, max_iter: int = 5000,
+                   rel_tol: float = 1e-10) -> MultiResult:
+    """nrhs simultaneous CG solves sharing one SpMM per iteration (row-major
+    n x nrhs B and X)."""
+    it = np.zeros(nrhs, np.int32)
+    rel = np.zeros(nrhs, np.float64)
+    sec = ctypes.c_double(0)
+    A.ctx.check(lib().mfe_cg_solve_multi(A.ctx.h, A.h, B.h, X.h, int(nrhs), int(max_iter),
+                                         float(rel_tol), _ptr(it), _ptr(rel), ctypes.byref(sec)),
+                "cg_solve_multi")
+    return MultiResult(it, rel, sec.value)
 
 
 class Batch:

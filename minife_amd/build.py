@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libminife_hip.so")
-SOURCES = ["mfe_api.hip", "mfe_assemble.hip", "mfe_solve.hip", "mfe_batch.hip"]
+SOURCES = ["mfe_api.hip", "mfe_assemble.hip", "mfe_solve.hip", "mfe_batch.hip", "mfe_multi.hip"]
 ARCH = os.environ.get("MFE_OFFLOAD_ARCH", "gfx950")
 
 

@@ -80,13 +80,15 @@ int mfe_ctx_create(int device, mfe_ctx** out) {
   return MFE_OK;
 }
 
-void mfe_cg_workspace_free(mfe_ctx* ctx);  // mfe_solve.hip
+void mfe_cg_workspace_free(mfe_ctx* ctx);     // mfe_solve.hip
+void mfe_multi_workspace_free(mfe_ctx* ctx);  // mfe_multi.hip
 
 void mfe_ctx_destroy(mfe_ctx* c) {
   if (!c) return;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   mfe_cg_workspace_free(c);
+  mfe_multi_workspace_free(c);
   if (c->d_partials) hipFree(c->d_partials);
   if (c->d_scalars) hipFree(c->d_scalars);
   if (c->h_scalars) hipHostFree(c->h_scalars);

@@ -22,7 +22,8 @@ constexpr int kSpmvMaxNnz = 27 * kSpmvRows;  // LDS product slots per workgroup
 constexpr int kVecBlocks = 2048;   // fixed grid of the vector kernels
 constexpr int kSellMaxWidth = 27;  // longest row the SELL SpMV takes (27-point stencil)
 
-struct CgWorkspace;  // mfe_solve.hip
+struct CgWorkspace;     // mfe_solve.hip
+struct MultiWorkspace;  // mfe_multi.hip
 
 }  // namespace mfe
 
@@ -36,6 +37,7 @@ struct mfe_ctx {
   double* d_scalars = nullptr;   // small device scalar area
   double* h_scalars = nullptr;   // pinned host mirror
   mfe::CgWorkspace* cg = nullptr;
+  mfe::MultiWorkspace* multi = nullptr;
   int cu_count = 0;
 };
 

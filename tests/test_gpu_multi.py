@@ -1,0 +1,129 @@
+"""GPU many right-hand sides: blocked SpMM and simultaneous CG.
+
+Checks: every SpMM tile gives, in column k, the single-vector SpMV's bits
+(the oracle's matvec) for nrhs = 4, 8, 16 on stencil and ragged matrices;
+the autotuner picks one of its timed tiles; the multi-RHS CG matches the
+oracle's per-column CG on a small mesh, and on the 128^3 workload (16
+right-hand sides, N(0,1) per free node, seed 1, zero on Dirichlet rows)
+every column reaches 1e-10 with the residual recomputed from the solution.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import bits_equal
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def mfe():
+    import minife_amd
+    minife_amd.lib()
+    return minife_amd
+
+
+def test_spmm_every_tile_bit_exact(mfe, oracle_mod):
+    for dims in [(17, 3, 5), (12, 10, 9)]:
+        P = oracle_mod.build_problem(*dims)
+        Ao = P["A"]
+        A = mfe.CSRMatrix.from_host(Ao.row_offsets, Ao.cols, Ao.vals)
+        n = Ao.nrows
+        for nrhs in (4, 8, 16):
+            X = np.random.default_rng(nrhs).standard_normal((n, nrhs))
+            want = np.stack([oracle_mod.matvec(Ao, np.ascontiguousarray(X[:, k]))
+                             for k in range(nrhs)], axis=1)
+            Xd = mfe.Vector.from_numpy(X.ravel())
+            Yd = mfe.Vector(n * nrhs)
+            tun = mfe.spmm_tune(A, nrhs)
+            say(f"spmm {dims} nrhs={nrhs}: tuned tile {tun['tile']} of {len(tun['ms'])}")
+            assert 0 <= tun["tile"] < len(tun["ms"])
+            for tile in list(range(len(tun["ms"]))) + [-1]:
+                mfe.spmm(A, Xd, Yd, nrhs, tile)
+                assert bits_equal(Yd.to_numpy().reshape(n, nrhs), want), (dims, nrhs, tile)
+
+
+def test_spmm_ragged_rows(mfe, oracle_mod):
+    rng = np.random.default_rng(9)
+    n = 777
+    lens = rng.integers(0, 40, n)  # includes rows longer than 27
+    ro = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    cols = np.concatenate([np.sort(rng.choice(n, size=int(l), replace=False)) for l in lens]).astype(np.int32)
+    vals = rng.standard_normal(cols.shape[0])
+    Ao = oracle_mod.CSR(np.arange(n, dtype=np.int32), ro, cols, vals)
+    A = mfe.CSRMatrix.from_host(ro, cols, vals)
+    X = rng.standard_normal((n, 8))
+    want = np.stack([oracle_mod.matvec(Ao, np.ascontiguousarray(X[:, k])) for k in range(8)], axis=1)
+    Xd, Yd = mfe.Vector.from_numpy(X.ravel()), mfe.Vector(n * 8)
+    for tile in range(len(mfe.spmm_tune(A, 8)["ms"])):
+        mfe.spmm(A, Xd, Yd, 8, tile)
+        assert bits_equal(Yd.to_numpy().reshape(n, 8), want), tile
+
+
+def rhs_block(oracle_mod, dims, nrhs, seed):
+    from minife_amd.workloads import multi_rhs_sources
+    n = (dims[0] + 1) * (dims[1] + 1) * (dims[2] + 1)
+    B = multi_rhs_sources(n, nrhs, seed)
+    bc = np.union1d(oracle_mod.bc_rows(*dims, 0), oracle_mod.bc_rows(*dims, 1))
+    B[bc, :] = 0.0
+    return B
+
+
+def test_multi_cg_matches_oracle(mfe, oracle_mod):
+    dims = (14, 12, 11)
+    say(f"multi cg {dims}")
+    P = oracle_mod.build_problem(*dims)
+    Ao = P["A"]
+    A = mfe.CSRMatrix.from_host(Ao.row_offsets, Ao.cols, Ao.vals)
+    mfe.make_local_matrix(A)
+    n = Ao.nrows
+    for nrhs in (4, 16):
+        B = rhs_block(oracle_mod, dims, nrhs, 1)
+        X = mfe.Vector(n * nrhs)
+        r = mfe.cg_solve_multi(A, mfe.Vector.from_numpy(B.ravel()), X, nrhs, max_iter=3000,
+                               rel_tol=1e-10)
+        Xh = X.to_numpy().reshape(n, nrhs)
+        assert np.all(r.rel_residual <= 1e-10)
+        for k in range(nrhs):
+            bk = np.ascontiguousarray(B[:, k])
+            bn = float(np.linalg.norm(bk))
+            ref = oracle_mod.cg_solve(Ao, bk, max_iter=3000, tol=1e-10 * bn)
+            assert abs(int(r.iters[k]) - ref["num_iters"]) <= 2, (k, r.iters[k], ref["num_iters"])
+            res = bk - oracle_mod.matvec(Ao, np.ascontiguousarray(Xh[:, k]))
+            assert np.linalg.norm(res) <= 4e-10 * bn
+            assert np.max(np.abs(Xh[:, k] - ref["x"])) <= 1e-8 * np.max(np.abs(ref["x"]))
+
+
+def test_multi_rhs_128(mfe, oracle_mod):
+    dims = (128, 128, 128)
+    nrhs = 16
+    say("multi 128^3 x 16")
+    mesh = mfe.simple_mesh_description([0, 128, 0, 128, 0, 128])
+    A = mfe.CSRMatrix()
+    assert mfe.generate_matrix_structure(mesh, A) == 0
+    b = mfe.Vector(A.nrows)
+    mfe.assemble_FE_data(mesh, A, b)
+    mfe.impose_dirichlet(0.0, A, b, 129, 129, 129, mesh.bc_rows_0)
+    mfe.impose_dirichlet(1.0, A, b, 129, 129, 129, mesh.bc_rows_1)
+    mfe.make_local_matrix(A)
+    n = A.nrows
+    B = rhs_block(oracle_mod, dims, nrhs, 1)
+    Bd = mfe.Vector.from_numpy(B.ravel())
+    tun = mfe.spmm_tune(A, nrhs)
+    say(f"  tuner: tile {tun['tile']} {tun['tiles'][tun['tile']]}, ms {np.round(tun['ms'], 4).tolist()}")
+    X = mfe.Vector(n * nrhs)
+    r = mfe.cg_solve_multi(A, Bd, X, nrhs, max_iter=5000, rel_tol=1e-10)
+    say(f"  {r.seconds:.3f} s, iters {r.iters.min()}..{r.iters.max()}")
+    assert np.all(r.rel_residual <= 1e-10)
+    Y = mfe.Vector(n * nrhs)
+    mfe.spmm(A, X, Y, nrhs)
+    R = B - Y.to_numpy().reshape(n, nrhs)
+    true_rel = np.linalg.norm(R, axis=0) / np.linalg.norm(B, axis=0)
+    say(f"  true rel residual max {true_rel.max():.2e}")
+    assert np.all(true_rel <= 4e-10)
