@@ -28,6 +28,7 @@
 
 #include "mfe_common.hpp"
 #include "mfe_hex8.hpp"
+#include "mfe_sell.hpp"
 
 namespace mfe {
 
@@ -178,7 +179,7 @@ __device__ double assemble_row(const BatchProb& P, int id, const double* K, cons
   // Store in ascending column order into the row's SELL slots; pad the rest.
   const int slice = id / kWave, lane = id % kWave;
   const int o = soff[slice], w = soff[slice + 1] - o;
-  const int64_t base = P.slot0 + (int64_t)o * kWave + lane;
+  const int64_t o64 = P.slot0 + (int64_t)o * kWave;
   int pos = 0;
 #pragma unroll
   for (int sz = -1; sz <= 1; ++sz)
@@ -189,13 +190,15 @@ __device__ double assemble_row(const BatchProb& P, int id, const double* K, cons
         const int s = (sz + 1) * 9 + (sy + 1) * 3 + (sx + 1);
         const bool in = sx >= lox && sx <= hix && sy >= loy && sy <= hiy && sz >= loz && sz <= hiz;
         if (!in) continue;
-        svals[base + (int64_t)pos * kWave] = acc[s];
-        scols[base + (int64_t)pos * kWave] = (ix + sx) + gx * ((iy + sy) + gy * (iz + sz));
+        const int64_t at = sell_pos(o64, pos, w, lane);
+        svals[at] = acc[s];
+        scols[at] = (ix + sx) + gx * ((iy + sy) + gy * (iz + sz));
         ++pos;
       }
   for (; pos < w; ++pos) {
-    svals[base + (int64_t)pos * kWave] = 0.0;
-    scols[base + (int64_t)pos * kWave] = id;
+    const int64_t at = sell_pos(o64, pos, w, lane);
+    svals[at] = 0.0;
+    scols[at] = id;
   }
   return accb;
 }
@@ -205,42 +208,26 @@ __device__ double assemble_row(const BatchProb& P, int id, const double* K, cons
 __device__ void pad_row(const BatchProb& P, int id, const int* soff, int* scols, double* svals) {
   const int slice = id / kWave, lane = id % kWave;
   const int o = soff[slice], w = soff[slice + 1] - o;
-  const int64_t base = P.slot0 + (int64_t)o * kWave + lane;
+  const int64_t o64 = P.slot0 + (int64_t)o * kWave;
   for (int j = 0; j < w; ++j) {
-    svals[base + (int64_t)j * kWave] = 0.0;
-    scols[base + (int64_t)j * kWave] = P.n - 1;
+    const int64_t at = sell_pos(o64, j, w, lane);
+    svals[at] = 0.0;
+    scols[at] = P.n - 1;
   }
 }
 
 // (A p)[row] for row < n of problem P from its SELL slices, summed over the
 // row's entries in stored column order (the row length is the box stencil's).
-__device__ __forceinline__ double sell_row_sum(const BatchProb& P, int row, const int* soff,
-                                               const int* scols, const double* svals,
-                                               const double* pv) {
+__device__ __forceinline__ double batch_row_sum(const BatchProb& P, int row, const int* soff,
+                                                const int* scols, const double* svals,
+                                                const double* pv) {
   const int slice = row / kWave, lane = row % kWave;
   const int o = soff[slice], w = soff[slice + 1] - o;
   const int gx = P.Nx + 1, gy = P.Ny + 1;
   const int rr = row < P.n ? row : P.n - 1;
   const int ix = rr % gx, iy = (rr / gx) % gy, iz = rr / (gx * gy);
   const int len = row < P.n ? axis_len(ix, gx) * axis_len(iy, gy) * axis_len(iz, P.Nz + 1) : 0;
-  double sum = 0.0;
-  if (w > 0) {
-    const int64_t base = P.slot0 + (int64_t)o * kWave + lane;
-    int c[kSellMaxWidth];
-    double v[kSellMaxWidth], xv[kSellMaxWidth];
-#pragma unroll
-    for (int j = 0; j < kSellMaxWidth; ++j) {
-      const int64_t at = base + (int64_t)(j < w ? j : w - 1) * kWave;
-      c[j] = scols[at];
-      v[j] = svals[at];
-    }
-#pragma unroll
-    for (int j = 0; j < kSellMaxWidth; ++j) xv[j] = pv[c[j]];
-#pragma unroll
-    for (int j = 0; j < kSellMaxWidth; ++j)
-      if (j < len) sum += v[j] * xv[j];
-  }
-  return sum;
+  return sell_row_sum<double>(scols, svals, pv, P.slot0 + (int64_t)o * kWave, w, lane, len, 0.0);
 }
 
 __global__ __launch_bounds__(kBlock) void batch_kernel(
@@ -326,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(
     for (int tile = i; tile < ntiles; tile += nb) {
       const int row = tile * kBlock + t;
       if (row / kWave < P.nslices) {
-        const double sum = sell_row_sum(P, row, soff, scols, svals, pp);
+        const double sum = batch_row_sum(P, row, soff, scols, svals, pp);
         if (row < P.n) {
           App[row] = sum;
           acc += sum * pp[row];
@@ -467,7 +454,7 @@ int mfe_batch_create(mfe_ctx* ctx, int nprob, const int* dims, mfe_batch** out) 
     B->soff.push_back(acc);
     row0 += P.n;
     soff0 += P.nslices + 1;
-    slot0 += (int64_t)acc * kWave;
+    slot0 += (int64_t)acc * kWave + kSellTail;  // each problem keeps its own over-read tail
     elem0 += 44 * (int64_t)P.ne;
     const double nnz = (3.0 * gx - 2) * (3.0 * gy - 2) * (3.0 * gz - 2);
     work[(size_t)q] = nnz * (double)(P.Nx + P.Ny + P.Nz);
@@ -530,6 +517,9 @@ int mfe_batch_create(mfe_ctx* ctx, int nprob, const int* dims, mfe_batch** out) 
   BALLOC(B->d_states, sizeof(BatchState) * (size_t)nprob);
   BALLOC(B->d_abort, sizeof(int));
 #undef BALLOC
+  // zero everything once: the per-problem tails stay value 0, column 0
+  MFE_CHECK(ctx, hipMemsetAsync(B->d_scols, 0, sizeof(int) * (size_t)B->slots, s));
+  MFE_CHECK(ctx, hipMemsetAsync(B->d_svals, 0, sizeof(double) * (size_t)B->slots, s));
   MFE_CHECK(ctx, hipMemcpyAsync(B->d_probs, B->probs.data(), sizeof(BatchProb) * (size_t)nprob,
                                 hipMemcpyHostToDevice, s));
   MFE_CHECK(ctx, hipMemcpyAsync(B->d_blk2prob, blk2prob.data(), sizeof(int) * blk2prob.size(),
@@ -640,7 +630,7 @@ int mfe_batch_download(mfe_ctx* ctx, const mfe_batch* B, int p, int* row_offsets
     if (row_offsets) row_offsets[id] = (int)pos;
     const int s = id / kWave, l = id % kWave;
     for (int j = 0; j < len; ++j, ++pos) {
-      const int64_t at = ((int64_t)so[s] + j) * kWave + l;
+      const int64_t at = sell_pos((int64_t)so[s] * kWave, j, so[s + 1] - so[s], l);
       if (cols) cols[pos] = sc[(size_t)at];
       if (vals) vals[pos] = sv[(size_t)at];
     }

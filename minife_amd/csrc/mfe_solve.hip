@@ -20,6 +20,7 @@
 #include <type_traits>
 
 #include "mfe_common.hpp"
+#include "mfe_sell.hpp"
 
 namespace mfe {
 
@@ -127,14 +128,12 @@ __global__ __launch_bounds__(kBlock) void spmv_rowwise_kernel(
 
 // ------------------------------------------------------------ SELL-64 -------
 //
-// Slice s holds rows 64s..64s+63; its width w_s is its longest row, and slot
-// j < w_s of lane l sits at ((off[s] + j) * 64 + l).  Lane l owns row 64s+l:
-// its value and column loads are 64 consecutive entries per slot (coalesced,
-// no LDS), and on a stencil matrix the x gathers of one slot hit consecutive
-// addresses too.  Slots past a row's length hold value 0 and the row's own
-// index and are skipped in the sum, which runs over the row's entries in
-// stored column order: the same bits as the CSR kernels.  Four slices per
-// workgroup, so the partials match the CSR kernel's (one per 256 rows).
+// Slice s holds rows 64s..64s+63 (one per lane) with width w_s = its longest
+// row, in the slot-pair layout of mfe_sell.hpp: coalesced 16-byte value and
+// 8-byte column loads, no LDS, and on a stencil matrix the x gathers of one
+// slot hit consecutive addresses.  Rows are summed in stored column order:
+// the same bits as the CSR kernels.  Four slices per workgroup, so the
+// partials match the CSR kernel's (one per 256 rows).
 
 constexpr int kSlice = kWave;
 constexpr int kSlicesPerBlock = kBlock / kSlice;
@@ -162,24 +161,7 @@ __global__ __launch_bounds__(kBlock) void spmv_sell_kernel(
     const int w = soff[slice + 1] - o;
     const int len = slen[row];
     if (ZERO_Y) sum = T(0) * y0;
-    if (w > 0) {
-      // All value/column loads first (slot index clamped into the slice, so
-      // every load is unconditional), then every x gather, then the sum.
-      const int64_t base = (int64_t)o * kSlice + t % kSlice;
-      int c[kSellMaxWidth];
-      T v[kSellMaxWidth], xv[kSellMaxWidth];
-#pragma unroll
-      for (int j = 0; j < kSellMaxWidth; ++j) {
-        const int64_t at = base + (int64_t)(j < w ? j : w - 1) * kSlice;
-        c[j] = cols[at];
-        v[j] = vals[at];
-      }
-#pragma unroll
-      for (int j = 0; j < kSellMaxWidth; ++j) xv[j] = x[c[j]];
-#pragma unroll
-      for (int j = 0; j < kSellMaxWidth; ++j)
-        if (j < len) sum += v[j] * xv[j];
-    }
+    sum = sell_row_sum<T>(cols, vals, x, (int64_t)o * kSlice, w, t % kSlice, len, sum);
     if (MODE == kResid) sum = T(1) * br + T(-1) * sum;
     if (valid) y[row] = sum;
   }
@@ -222,7 +204,7 @@ __global__ __launch_bounds__(kBlock) void sell_fill_kernel(
   const int start = row < n ? ro[row] : 0;
   const int pad = row < n ? row : n - 1;
   for (int j = 0; j < w; ++j) {
-    const int64_t at = ((int64_t)o + j) * kSlice + row % kSlice;
+    const int64_t at = sell_pos((int64_t)o * kSlice, j, w, row % kSlice);
     const bool in = j < len;
     if (scols) scols[at] = in ? cols[start + j] : pad;
     if (svals) svals[at] = in ? (V)vals[start + j] : V(0);
@@ -309,8 +291,9 @@ static int ensure_plan(mfe_ctx* ctx, const mfe_csr* Ac, bool need64, bool need32
       A->sell_nslices = ns;
       A->sell_slots = acc * kSlice;
       ok = maxw <= kSellMaxWidth && acc < INT32_MAX &&
-           try_alloc(&A->sell_cols, sizeof(int) * (size_t)(A->sell_slots > 0 ? A->sell_slots : 1));
+           try_alloc(&A->sell_cols, sizeof(int) * (size_t)(A->sell_slots + kSellTail));
       if (ok) {
+        MFE_CHECK(ctx, hipMemsetAsync(A->sell_cols + A->sell_slots, 0, sizeof(int) * kSellTail, s));
         MFE_CHECK(ctx, hipMemcpyAsync(A->sell_off, off.data(), sizeof(int) * (size_t)(ns + 1),
                                       hipMemcpyHostToDevice, s));
         sell_fill_kernel<double><<<(unsigned)cdiv(ns * kSlice, kBlock), kBlock, 0, s>>>(
@@ -330,9 +313,12 @@ static int ensure_plan(mfe_ctx* ctx, const mfe_csr* Ac, bool need64, bool need32
   const bool sell = A->spmv_format == MFE_SPMV_AUTO && A->sell_ok && n > 0;
   const unsigned fg = (unsigned)cdiv(A->sell_nslices * kSlice, kBlock);
   if (sell && need64 && !A->sell_vals_valid) {
-    if (!A->sell_vals && !try_alloc(&A->sell_vals, sizeof(double) * (size_t)std::max<int64_t>(A->sell_slots, 1))) {
-      sell_disable(A);  // no room for the copy: CSR kernels
-      return ensure_plan(ctx, A, need64, need32);
+    if (!A->sell_vals) {
+      if (!try_alloc(&A->sell_vals, sizeof(double) * (size_t)(A->sell_slots + kSellTail))) {
+        sell_disable(A);  // no room for the copy: CSR kernels
+        return ensure_plan(ctx, A, need64, need32);
+      }
+      MFE_CHECK(ctx, hipMemsetAsync(A->sell_vals + A->sell_slots, 0, sizeof(double) * kSellTail, s));
     }
     sell_fill_kernel<double><<<fg, kBlock, 0, s>>>((int)n, (int)A->sell_nslices, A->sell_off,
                                                    A->sell_len, A->row_offsets, A->cols, A->vals,
@@ -341,9 +327,12 @@ static int ensure_plan(mfe_ctx* ctx, const mfe_csr* Ac, bool need64, bool need32
     A->sell_vals_valid = true;
   }
   if (sell && need32 && !A->sell_vals32_valid) {
-    if (!A->sell_vals32 && !try_alloc(&A->sell_vals32, sizeof(float) * (size_t)std::max<int64_t>(A->sell_slots, 1))) {
-      sell_disable(A);
-      return ensure_plan(ctx, A, need64, need32);
+    if (!A->sell_vals32) {
+      if (!try_alloc(&A->sell_vals32, sizeof(float) * (size_t)(A->sell_slots + kSellTail))) {
+        sell_disable(A);
+        return ensure_plan(ctx, A, need64, need32);
+      }
+      MFE_CHECK(ctx, hipMemsetAsync(A->sell_vals32 + A->sell_slots, 0, sizeof(float) * kSellTail, s));
     }
     sell_fill_kernel<float><<<fg, kBlock, 0, s>>>((int)n, (int)A->sell_nslices, A->sell_off,
                                                   A->sell_len, A->row_offsets, A->cols, A->vals,

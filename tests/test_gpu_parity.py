@@ -260,6 +260,29 @@ def test_sell_and_csr_kernels_bit_exact_on_ragged_rows(mfe, ctx, oracle_mod, n):
         assert bits_equal(y2.to_numpy(), want32), kernel
 
 
+@pytest.mark.parametrize("width", [1, 2, 3, 5, 8, 26, 27])
+def test_sell_uniform_slice_widths(mfe, ctx, oracle_mod, width):
+    """Every slice of one width (odd widths keep a single last slot; a width-1
+    slice's pair loads reach past the slice), 130 rows = 2 full slices + 2 rows."""
+    say(f"case sell width {width}")
+    rng = np.random.default_rng(width)
+    n = 130
+    ro = (np.arange(n + 1) * width).astype(np.int32)
+    cols = np.concatenate([np.sort(rng.choice(n, size=width, replace=False))
+                           for _ in range(n)]).astype(np.int32)
+    vals = rng.standard_normal(cols.shape[0])
+    Ao = oracle_mod.CSR(np.arange(n, dtype=np.int32), ro, cols, vals)
+    A = mfe.CSRMatrix.from_host(ro, cols, vals, ctx=ctx)
+    assert mfe.spmv_plan(A)["kernel"] == "sell64"
+    xh = rng.standard_normal(n)
+    x = mfe.Vector.from_numpy(xh, ctx)
+    y = mfe.Vector(n, ctx)
+    mfe.matvec(A, x, y)
+    assert bits_equal(y.to_numpy(), oracle_mod.matvec(Ao, xh))
+    mfe.matvec_fp32(A, x, y)
+    assert bits_equal(y.to_numpy(), oracle_mod.matvec_f32(Ao, xh))
+
+
 def test_sell_plan_follows_value_and_structure_changes(mfe, ctx, oracle_mod):
     """The SELL copy is rebuilt after the values change (element loop,
     Dirichlet) and after a new structure is generated into the same matrix."""
