@@ -86,6 +86,93 @@ def cpu_baseline(A_dev, b_dev, n, nnz, iters: int):
     }
 
 
+def bench_batched(mfe, ctx):
+    """64 boxes, sides uniform on [16, 96] (seed 0): assembly + FP64 CG to
+    1e-10 in one persistent launch.  Equivalent bytes per problem and
+    iteration: 12 B per stored SELL entry + 88 B per row (p update, x gather,
+    Ap, x/r update), plus nothing for assembly (not counted)."""
+    from minife_amd.workloads import batch_boxes
+    B = mfe.Batch(batch_boxes(0), ctx)
+    B.solve(max_iter=5000, rel_tol=1e-10)  # warm (code objects, first touch)
+    out = B.solve(max_iter=5000, rel_tol=1e-10)
+    byts = 0
+    for p in range(B.nprob):
+        inf = B.info(p)
+        byts += int(out["iters"][p]) * (12 * inf["stored"] + 88 * inf["rows"])
+    tot = B.info()
+    res = {"workload": "64 boxes, sides U[16,96] seed 0, assemble + FP64 CG to 1e-10, one launch",
+           "seconds": out["seconds"], "problems": B.nprob, "workgroups": tot["workgroups"],
+           "rows": tot["rows"], "nnz": tot["nnz"], "iters_total": int(out["iters"].sum()),
+           "iters_max": int(out["iters"].max()),
+           "max_rel_residual": float(out["rel_residual"].max()),
+           "cg_equiv_GBs": byts / out["seconds"] / 1e9}
+    B.close()
+    return res
+
+
+def bench_multi_rhs(mfe, ctx):
+    """128^3 with 16 right-hand sides (N(0,1) per free node, seed 1): simultaneous
+    CG over the autotuned CSR x 16 SpMM, against 16 single-vector solves."""
+    from minife_amd.workloads import multi_rhs_sources
+    N, K = 128, 16
+    mesh = mfe.simple_mesh_description([0, N, 0, N, 0, N])
+    A = mfe.CSRMatrix(ctx)
+    assert mfe.generate_matrix_structure(mesh, A) == 0
+    b = mfe.Vector(A.nrows, ctx)
+    mfe.assemble_FE_data(mesh, A, b)
+    mfe.impose_dirichlet(0.0, A, b, N + 1, N + 1, N + 1, mesh.bc_rows_0)
+    mfe.impose_dirichlet(1.0, A, b, N + 1, N + 1, N + 1, mesh.bc_rows_1)
+    mfe.make_local_matrix(A)
+    n, nnz = A.shape
+    Bh = multi_rhs_sources(n, K, 1)
+    Bh[np.union1d(mesh.bc_rows_0, mesh.bc_rows_1), :] = 0.0
+    Bd = mfe.Vector.from_numpy(Bh.ravel(), ctx)
+    tun = mfe.spmm_tune(A, K)
+    X = mfe.Vector(n * K, ctx)
+    mfe.cg_solve_multi(A, Bd, X, K, max_iter=5000, rel_tol=1e-10)  # warm (graph capture)
+    X.fill(0.0)
+    r = mfe.cg_solve_multi(A, Bd, X, K, max_iter=5000, rel_tol=1e-10)
+    # the same 16 systems one at a time with the single-vector CG
+    t_single = 0.0
+    its_single = 0
+    xs = mfe.Vector(n, ctx)
+    for k in range(K):
+        bk = mfe.Vector.from_numpy(np.ascontiguousarray(Bh[:, k]), ctx)
+        xs.fill(0.0)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        rk = mfe.cg_solve(A, bk, xs, max_iter=5000, tolerance=1e-10, relative=True)
+        t_single += time.perf_counter() - t0
+        its_single += rk.num_iters
+    ms = tun["ms"][tun["tile"]]
+    spmm_bytes = 12 * nnz + 4 * (n + 1) + 2 * 8 * K * n
+    return {"workload": "128^3, 16 RHS N(0,1) per free node seed 1, FP64 CG to 1e-10 per column",
+            "seconds": r.seconds, "iters_min": int(r.iters.min()), "iters_max": int(r.iters.max()),
+            "max_rel_residual": float(r.rel_residual.max()),
+            "single_rhs_seconds": t_single, "single_rhs_iters": its_single,
+            "speedup_vs_single": t_single / r.seconds,
+            "spmm_tile": tun["tiles"][tun["tile"]], "spmm_ms": ms,
+            "spmm_GBs": spmm_bytes / (ms * 1e-3) / 1e9,
+            "tuner_ms": [round(v, 4) for v in tun["ms"]]}
+
+
+def bench_adjoint(mfe):
+    """128^3 differentiable solve: loss and dL/dkappa (forward CG, adjoint CG,
+    element-gradient kernel), kappa = exp(N(0, 0.5^2)) seed 2, target from seed 3."""
+    from minife_amd import adjoint
+    from minife_amd.workloads import kappa_field
+    N = 128
+    ne = N ** 3
+    ut = adjoint.solve_u(N, N, N, kappa_field(ne, 3))
+    kappa = kappa_field(ne, 2)
+    adjoint.loss_and_gradient(N, N, N, kappa, ut)  # warm
+    t0 = time.perf_counter()
+    L, g, _ = adjoint.loss_and_gradient(N, N, N, kappa, ut)
+    dt = time.perf_counter() - t0
+    return {"workload": "128^3 kappa=exp(N(0,0.25)) seed 2, L=0.5||u-u_t||^2, FP64, rel tol 1e-13",
+            "seconds": dt, "loss": L, "grad_norm": float(np.linalg.norm(g))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +184,10 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tol-run", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the batched, multi-RHS and adjoint workloads")
+    ap.add_argument("--only", choices=["batched", "multi_rhs", "adjoint"],
+                    help="run just this extra workload and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -108,6 +199,12 @@ def main():
     ctx = mfe.Context(0)
     info = ctx.device_info()
     log(f"device {info}")
+    if args.only:
+        fn = {"batched": lambda: bench_batched(mfe, ctx),
+              "multi_rhs": lambda: bench_multi_rhs(mfe, ctx),
+              "adjoint": lambda: bench_adjoint(mfe)}[args.only]
+        print(json.dumps({args.only: fn(), "device": info["name"]}), flush=True)
+        return
     N = args.n
     ne = N ** 3
 
@@ -214,6 +311,16 @@ def main():
             f"|x_mixed-x_fp64|/|x_fp64| = {err:.2e}")
         del res
 
+    extras = {}
+    if not args.no_extras:
+        del x
+        extras["batched"] = bench_batched(mfe, ctx)
+        log(f"batched: {extras['batched']}")
+        extras["multi_rhs"] = bench_multi_rhs(mfe, ctx)
+        log(f"multi-rhs: {extras['multi_rhs']}")
+        extras["adjoint"] = bench_adjoint(mfe)
+        log(f"adjoint: {extras['adjoint']}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -255,6 +362,7 @@ def main():
                      "elements_per_s": ne / t_asm},
         "time_to_1e-10": tol_run,
         "cpu_baseline": cpu,
+        **extras,
         "device": info["name"],
     }
     print(json.dumps(out), flush=True)
