@@ -295,20 +295,40 @@ def main():
             rm = mfe.cg_solve_mixed(A, b, xm, inner_tol=1e-5, outer_tol=1e-10)
             tm = time.perf_counter() - t0
         xmh = xm.to_numpy()
-        # both residuals recomputed in FP64 on the GPU
+        # mixed, keeping the FP32 CG's direction across refinements
+        xk = mfe.Vector(n, ctx)
+        for timed in (False, True):
+            xk.fill(0.0)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            rk = mfe.cg_solve_mixed(A, b, xk, inner_tol=1e-5, outer_tol=1e-10,
+                                    keep_direction=True)
+            tk = time.perf_counter() - t0
+        xkh = xk.to_numpy()
+        # all residuals recomputed in FP64 on the GPU
         res = mfe.Vector(n, ctx)
+        bh = b.to_numpy()
         mfe.matvec(A, x, res)
-        rel64 = float(np.linalg.norm(b.to_numpy() - res.to_numpy())) / bnorm
+        rel64 = float(np.linalg.norm(bh - res.to_numpy())) / bnorm
         mfe.matvec(A, xm, res)
-        relm = float(np.linalg.norm(b.to_numpy() - res.to_numpy())) / bnorm
+        relm = float(np.linalg.norm(bh - res.to_numpy())) / bnorm
+        mfe.matvec(A, xk, res)
+        relk = float(np.linalg.norm(bh - res.to_numpy())) / bnorm
         err = float(np.linalg.norm(xmh - x64) / np.linalg.norm(x64))
+        errk = float(np.linalg.norm(xkh - x64) / np.linalg.norm(x64))
         tol_run = {"fp64_s": tt, "fp64_iters": r2.num_iters, "fp64_true_rel_residual": rel64,
                    "mixed_s": tm, "mixed_outer": rm.outer_iters, "mixed_inner": rm.inner_iters,
                    "mixed_true_rel_residual": relm, "mixed_converged": rm.converged,
-                   "mixed_rel_err_vs_fp64": err, "speedup_mixed_vs_fp64": tt / tm}
+                   "mixed_rel_err_vs_fp64": err, "speedup_mixed_vs_fp64": tt / tm,
+                   "mixed_keep_direction": {
+                       "s": tk, "outer": rk.outer_iters, "inner": rk.inner_iters,
+                       "true_rel_residual": relk, "converged": rk.converged,
+                       "rel_err_vs_fp64": errk, "speedup_vs_fp64": tt / tk}}
         log(f"time to 1e-10: fp64 {tt:.3f}s ({r2.num_iters} its, true rel {rel64:.2e}); mixed "
             f"{tm:.3f}s ({rm.outer_iters} outer, {rm.inner_iters} inner, true rel {relm:.2e}), "
-            f"|x_mixed-x_fp64|/|x_fp64| = {err:.2e}")
+            f"|x_mixed-x_fp64|/|x_fp64| = {err:.2e}; keep-direction {tk:.3f}s ({rk.outer_iters} "
+            f"outer, {rk.inner_iters} inner, true rel {relk:.2e}, err {errk:.2e})")
+        del xk
         del res
 
     extras = {}

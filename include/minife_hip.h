@@ -206,6 +206,15 @@ int mfe_cg_solve_mixed(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec
 /* y = A x evaluated as the mixed solve's inner SpMV: FP32 values, x rounded
  * to FP32, FP32 products summed in stored column order, y widened to FP64. */
 int mfe_matvec_fp32(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y);
+/* mfe_cg_solve_mixed with flags.  MFE_MIXED_KEEP_DIRECTION: the inner FP32 CG
+ * is not restarted at a refinement; its residual is replaced by the rounded
+ * FP64 residual b - A x and it continues with its search direction (reliable
+ * updates), so the inner iterations total about what FP64 CG needs. */
+#define MFE_MIXED_KEEP_DIRECTION 0x1u
+int mfe_cg_solve_mixed_ex(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x,
+                          int max_outer, int max_inner, double inner_tol, double outer_tol,
+                          unsigned flags, int* outer_iters, int* inner_iters,
+                          double* rel_residual, double* times);
 
 /* Kernel timing mode, the counterpart of time_kernels (time_kernels.hpp:42-134):
  * launches `reps` back-to-back SpMV kernels y = A x on ctx's stream between two

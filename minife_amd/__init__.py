@@ -47,7 +47,7 @@ EXPORTS = [
     "mfe_generate_matrix_structure", "mfe_perform_element_loop", "mfe_assemble_FE_data",
     "mfe_impose_dirichlet", "mfe_make_local_matrix", "mfe_spmv_plan",
     "mfe_matvec", "mfe_matvec_and_dot", "mfe_dot", "mfe_dot_replay", "mfe_waxpby",
-    "mfe_cg_solve", "mfe_cg_solve_mixed", "mfe_matvec_fp32", "mfe_time_matvec", "mfe_soln",
+    "mfe_cg_solve", "mfe_cg_solve_mixed", "mfe_cg_solve_mixed_ex", "mfe_matvec_fp32", "mfe_time_matvec", "mfe_soln",
     "mfe_verify_solution", "mfe_batch_create", "mfe_batch_destroy", "mfe_batch_info",
     "mfe_batch_solve", "mfe_batch_download", "mfe_spmm", "mfe_spmm_tune", "mfe_spmm_tile_info",
     "mfe_cg_solve_multi",
@@ -142,6 +142,10 @@ def lib() -> ctypes.CDLL:
                                 ctypes.c_double, ctypes.POINTER(ctypes.c_int),
                                 ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
                                 _vp], ctypes.c_int),
+        "mfe_cg_solve_mixed_ex": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                   ctypes.c_double, ctypes.c_uint, ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                                   _vp], ctypes.c_int),
         "mfe_matvec_fp32": ([_vp, _vp, _vp, _vp], ctypes.c_int),
         "mfe_time_matvec": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int,
                              ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
@@ -509,17 +513,24 @@ class MixedResult:
         return self.status == MFE_OK
 
 
+MIXED_KEEP_DIRECTION = 0x1
+
+
 def cg_solve_mixed(A: CSRMatrix, b: Vector, x: Vector, *, max_outer: int = 20,
                    max_inner: int = 2000, inner_tol: float = 1e-5,
-                   outer_tol: float = 1e-10) -> MixedResult:
+                   outer_tol: float = 1e-10, keep_direction: bool = False) -> MixedResult:
     """Mixed-precision iterative refinement around cg_solve (FP32 inner CG on an
-    FP32 copy of A, FP64 residual/update/norms on the GPU); see minife_hip.h."""
+    FP32 copy of A, FP64 residual/update/norms on the GPU).  keep_direction:
+    the inner CG continues across refinements with its residual replaced by
+    the FP64 one (reliable updates) instead of restarting."""
     oi, ii = ctypes.c_int(0), ctypes.c_int(0)
     rel = ctypes.c_double(0)
     times = np.zeros(5, np.float64)
-    st = lib().mfe_cg_solve_mixed(A.ctx.h, A.h, b.h, x.h, int(max_outer), int(max_inner),
-                                  float(inner_tol), float(outer_tol), ctypes.byref(oi),
-                                  ctypes.byref(ii), ctypes.byref(rel), _ptr(times))
+    st = lib().mfe_cg_solve_mixed_ex(A.ctx.h, A.h, b.h, x.h, int(max_outer), int(max_inner),
+                                     float(inner_tol), float(outer_tol),
+                                     MIXED_KEEP_DIRECTION if keep_direction else 0,
+                                     ctypes.byref(oi), ctypes.byref(ii), ctypes.byref(rel),
+                                     _ptr(times))
     if st not in (MFE_OK, MFE_BREAKDOWN, MFE_UNSUPPORTED):
         A.ctx.check(st, "cg_solve_mixed")
     return MixedResult(oi.value, ii.value, rel.value, st, float(times[T_TOTAL]))

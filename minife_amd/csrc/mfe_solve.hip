@@ -707,6 +707,31 @@ __global__ __launch_bounds__(kBlock) void cg_next_fast_kernel(CgState* st, const
   if (threadIdx.x == 0) cg_next(st, (double)rr);
 }
 
+// Continue a stopped CG after its residual was replaced (the mixed solve's
+// FP64 residual, rounded): rtrans = r.r of the new residual, beta = rtrans /
+// rtrans of the last iteration, so the next cg_p forms p = r + beta p with
+// the kept direction.  k restarts at 2 (history rows stay in range).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_resume_kernel(CgState* st, const T* partials, int np,
+                                                           int max_iter, double tol) {
+  __shared__ T red[4];
+  const T rr = sum_partials<T>(partials, np, red);
+  if (threadIdx.x == 0) {
+    st->oldrtrans = st->rtrans;
+    st->rtrans = (double)rr;
+    st->beta = st->oldrtrans > 0 ? st->rtrans / st->oldrtrans : 0.0;
+    st->normr = sqrt(st->rtrans);
+    st->k = 1;
+    st->max_iter = max_iter;
+    st->tol = tol;
+    st->done = !(st->normr > tol) || max_iter < 1;
+    st->status = 0;
+    st->num_iters = 0;
+    st->need_norms = 0;
+    st->x_pending = 0;
+  }
+}
+
 // Replay mode scalar kernels (sequential dots, FP64 only) -------------------------
 __global__ __launch_bounds__(kBlock) void cg_first_seq_kernel(CgState* st, int64_t n,
                                                               const double* r) {
@@ -1110,6 +1135,59 @@ static int run_cg(mfe_ctx* ctx, const SpmvOp<T>& op, const T* b, T* x, int max_i
   return MFE_OK;
 }
 
+// Resume the stopped CG held in w->st with v.r just replaced and x (the
+// correction) zeroed by the caller: up to max_iter more iterations to
+// ||r|| <= tol.  Uses the same captured chunk as run_cg's k >= 2 iterations.
+template <typename T>
+static int run_cg_resume(mfe_ctx* ctx, const SpmvOp<T>& op, T* x, int max_iter, double tol,
+                         CgWorkspace* w, VecSet<T>& v, GraphSlot& slot, const void* key_b,
+                         const void* key_x) {
+  hipStream_t s = ctx->stream;
+  const int64_t n = v.n;
+  if (n == 0) return MFE_OK;
+  const unsigned vg = vec_grid(n);
+  dot_partial_kernel<T><<<vg, kBlock, 0, s>>>(n, v.r, v.r, v.partials);
+  cg_resume_kernel<T><<<1, kBlock, 0, s>>>(w->st, v.partials, (int)vg, max_iter, tol);
+  MFE_LAUNCHED(ctx, "cg_resume_kernel");
+  const bool reuse = slot.exec && slot.kA == op.A && slot.kgen == op.gen &&
+                     slot.kvals == op.key() && slot.kb == key_b && slot.kx == key_x &&
+                     slot.kflags == 0;
+  if (!reuse) {
+    slot.drop();
+    MFE_CHECK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t ce = hipSuccess;
+    for (int c = 0; c < kGraphChunk && ce == hipSuccess; ++c)
+      ce = enqueue_iteration<T>(op, v, w->st, w->hist, x, v.p, false, s);
+    hipGraph_t g = nullptr;
+    hipError_t ee = hipStreamEndCapture(s, &g);
+    if (ce != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return hip_fail(ctx, ce, "cg capture");
+    }
+    if (ee != hipSuccess) return hip_fail(ctx, ee, "hipStreamEndCapture");
+    slot.graph = g;
+    MFE_CHECK(ctx, hipGraphInstantiate(&slot.exec, g, nullptr, nullptr, 0));
+    slot.kA = op.A;
+    slot.kgen = op.gen;
+    slot.kvals = op.key();
+    slot.kb = key_b;
+    slot.kx = key_x;
+    slot.kflags = 0;
+  }
+  const int chunks = (int)cdiv(std::max(max_iter, 1), kGraphChunk);
+  for (int c = 0; c < chunks; ++c) {
+    MFE_CHECK(ctx, hipGraphLaunch(slot.exec, s));
+    MFE_CHECK(ctx, hipMemcpyAsync(w->h_st, w->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    MFE_CHECK(ctx, hipStreamSynchronize(s));
+    if (w->h_st->done) break;
+  }
+  cg_finish_kernel<T><<<vg, kBlock, 0, s>>>(n, w->st, v.p, x);
+  MFE_LAUNCHED(ctx, "cg_finish_kernel");
+  MFE_CHECK(ctx, hipMemcpyAsync(w->h_st, w->st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+  MFE_CHECK(ctx, hipStreamSynchronize(s));
+  return MFE_OK;
+}
+
 }  // namespace mfe
 
 using namespace mfe;
@@ -1363,11 +1441,13 @@ int mfe_cg_solve(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x, i
   return hs.status == MFE_BREAKDOWN ? MFE_BREAKDOWN : MFE_OK;
 }
 
-int mfe_cg_solve_mixed(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x,
-                       int max_outer, int max_inner, double inner_tol, double outer_tol,
-                       int* outer_iters, int* inner_iters, double* rel_residual, double* times) {
+int mfe_cg_solve_mixed_ex(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x,
+                          int max_outer, int max_inner, double inner_tol, double outer_tol,
+                          unsigned flags, int* outer_iters, int* inner_iters,
+                          double* rel_residual, double* times) {
   if (!ctx || !A || !outer_iters || !inner_iters || !rel_residual || max_outer < 0 || max_inner < 0)
     return MFE_BAD_ARG;
+  const bool keep = (flags & MFE_MIXED_KEEP_DIRECTION) != 0;
   if (!A->has_local_indices)
     return fail(ctx, MFE_NOT_LOCAL,
                 "cg_solve_mixed: A.has_local_indices is false; call make_local_matrix(A) first");
@@ -1432,12 +1512,25 @@ int mfe_cg_solve_mixed(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec
     // ahead of the true one by a small factor).
     const double need = 0.1 * outer_tol / rel;
     const double tol_k = need > inner_tol ? std::min(need, 0.1) : inner_tol;
-    to_f32_kernel<<<vg, kBlock, 0, s>>>(n, v.r, 1.0 / rn, w->f_rhs);
+    // Keeping the direction: the inner system is A d = r itself (no scaling,
+    // so the kept p and rtrans stay in the same units), and every refinement
+    // after the first resumes the stopped FP32 CG with r replaced by the
+    // rounded FP64 residual (reliable updates).
+    const double scale = keep ? 1.0 : rn;
     MFE_CHECK(ctx, hipMemsetAsync(w->f_x, 0, sizeof(float) * (size_t)n, s));
-    MFE_LAUNCHED(ctx, "to_f32_kernel");
-    if ((st = run_cg<float>(ctx, op32, w->f_rhs, w->f_x, max_inner, tol_k, false,
-                            false, w, w->v32, w->g32, w->f_rhs, w->f_x, 0)))
-      return st;
+    if (keep && outer > 0) {
+      to_f32_kernel<<<vg, kBlock, 0, s>>>(n, v.r, 1.0, w->v32.r);
+      MFE_LAUNCHED(ctx, "to_f32_kernel");
+      if ((st = run_cg_resume<float>(ctx, op32, w->f_x, max_inner, tol_k * rn, w, w->v32, w->g32,
+                                     w->f_rhs, w->f_x)))
+        return st;
+    } else {
+      to_f32_kernel<<<vg, kBlock, 0, s>>>(n, v.r, 1.0 / scale, w->f_rhs);
+      MFE_LAUNCHED(ctx, "to_f32_kernel");
+      if ((st = run_cg<float>(ctx, op32, w->f_rhs, w->f_x, max_inner, tol_k * (keep ? rn : 1.0),
+                              false, false, w, w->v32, w->g32, w->f_rhs, w->f_x, 0)))
+        return st;
+    }
     const CgState hs = *w->h_st;
     inner_total += hs.num_iters;
     if (hs.num_iters == 0) {
@@ -1445,7 +1538,7 @@ int mfe_cg_solve_mixed(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec
       fail(ctx, MFE_BREAKDOWN, "cg_solve_mixed: inner solve made no progress");
       break;
     }
-    refine_update_kernel<<<vg, kBlock, 0, s>>>(n, rn, w->f_x, x->d);
+    refine_update_kernel<<<vg, kBlock, 0, s>>>(n, scale, w->f_x, x->d);
     MFE_LAUNCHED(ctx, "refine_update_kernel");
     ++outer;
     if ((st = residual(&rn))) return st;
@@ -1468,6 +1561,13 @@ int mfe_cg_solve_mixed(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec
     times[MFE_T_TOTAL] = 1e-3 * ms;
   }
   return status;
+}
+
+int mfe_cg_solve_mixed(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x,
+                       int max_outer, int max_inner, double inner_tol, double outer_tol,
+                       int* outer_iters, int* inner_iters, double* rel_residual, double* times) {
+  return mfe_cg_solve_mixed_ex(ctx, A, b, x, max_outer, max_inner, inner_tol, outer_tol, 0u,
+                               outer_iters, inner_iters, rel_residual, times);
 }
 
 }  // extern "C"

@@ -370,6 +370,32 @@ def test_mixed_precision_refinement(mfe, ctx, dims):
     assert mfe.verify_solution(mesh, x)[0] == 0
 
 
+@pytest.mark.parametrize("dims", [(30, 30, 30), (64, 48, 40)])
+def test_mixed_keep_direction(mfe, ctx, dims):
+    """Reliable-update refinement (the FP32 CG keeps its direction across
+    refinements): same FP64-verified residual as the restarted form, with no
+    more inner iterations."""
+    say(f"case mixed keep-direction {dims}")
+    mesh, A, b, _ = build_stages(mfe, ctx, dims)
+    x = mfe.Vector(A.nrows, ctx)
+    rk = mfe.cg_solve_mixed(A, b, x, inner_tol=1e-5, outer_tol=1e-10, keep_direction=True)
+    assert rk.converged and rk.rel_residual <= 1e-10
+    y = mfe.Vector(A.nrows, ctx)
+    mfe.matvec(A, x, y)
+    bh = b.to_numpy()
+    assert np.linalg.norm(bh - y.to_numpy()) / np.linalg.norm(bh) <= 1e-10
+    x2 = mfe.Vector(A.nrows, ctx)
+    rr = mfe.cg_solve_mixed(A, b, x2, inner_tol=1e-5, outer_tol=1e-10)
+    x64 = mfe.Vector(A.nrows, ctx)
+    r64 = mfe.cg_solve(A, b, x64, max_iter=5000, tolerance=1e-10, relative=True)
+    say(f"keep: outer {rk.outer_iters} inner {rk.inner_iters}; restart: outer {rr.outer_iters} "
+        f"inner {rr.inner_iters}; fp64 {r64.num_iters}")
+    assert rk.inner_iters <= rr.inner_iters
+    xf = x64.to_numpy()
+    assert np.linalg.norm(x.to_numpy() - xf) / np.linalg.norm(xf) <= 1e-7
+    assert mfe.verify_solution(mesh, x)[0] == 0
+
+
 def test_upload_rejects_bad_columns(mfe, ctx):
     with pytest.raises(mfe.MfeError) as e:
         mfe.CSRMatrix.from_host([0, 1, 2], [0, 5], [1.0, 1.0], ctx=ctx)
