@@ -49,7 +49,7 @@ EXPORTS = [
     "mfe_matvec", "mfe_matvec_and_dot", "mfe_dot", "mfe_dot_replay", "mfe_waxpby",
     "mfe_cg_solve", "mfe_cg_solve_mixed", "mfe_cg_solve_mixed_ex", "mfe_matvec_fp32", "mfe_time_matvec", "mfe_soln",
     "mfe_verify_solution", "mfe_batch_create", "mfe_batch_destroy", "mfe_batch_info",
-    "mfe_batch_solve", "mfe_batch_download", "mfe_spmm", "mfe_spmm_tune", "mfe_spmm_tile_info",
+    "mfe_batch_solve", "mfe_batch_download", "mfe_batch_timing", "mfe_spmm", "mfe_spmm_tune", "mfe_spmm_tile_info",
     "mfe_cg_solve_multi",
 ]
 
@@ -164,6 +164,7 @@ def lib() -> ctypes.CDLL:
         "mfe_batch_solve": ([_vp, _vp, ctypes.c_int, ctypes.c_double, _vp, _vp,
                              ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "mfe_batch_download": ([_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
+        "mfe_batch_timing": ([_vp, _vp], ctypes.c_int),
         "mfe_verify_solution": ([_vp, _int6, _vp, ctypes.c_double, ctypes.c_int,
                                  ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     }
@@ -630,7 +631,10 @@ class Batch:
                                    _ptr(rel), ctypes.byref(sec))
         if st not in (MFE_OK, MFE_BREAKDOWN):
             self.ctx.check(st, "batch_solve")
-        return dict(iters=iters, rel_residual=rel, seconds=sec.value, status=st)
+        per = np.zeros(self.nprob, np.float64)
+        lib().mfe_batch_timing(self.h, _ptr(per))
+        return dict(iters=iters, rel_residual=rel, seconds=sec.value, status=st,
+                    problem_seconds=per)
 
     def download(self, p: int, matrix: bool = True) -> dict:
         inf = self.info(p)

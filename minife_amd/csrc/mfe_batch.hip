@@ -49,18 +49,22 @@ struct BatchState {  // per problem, zeroed before each launch
   unsigned count, gen;
   int iters, status;  // status: 0 converged or max_iter, 1 breakdown, 2 aborted
   double normr, tol, bnorm, pad;
+  uint64_t t_begin, t_end;  // wall clock of the problem's first workgroup
 };
 
 constexpr double kBatchWaitSeconds = 10.0;
 
-// Problem-local barrier.  Every thread fences (release of its own stores),
-// thread 0 arrives and waits for the generation to move, every thread fences
-// again (acquire).  Returns false when the batch was aborted.
+// Problem-local barrier.  The workgroup's stores are complete in L2 after
+// __syncthreads (L1 writes through); thread 0 then releases them at agent
+// scope (writes the L2 back, so workgroups on other XCDs see them), arrives,
+// waits for the generation to move, and acquires (invalidates this CU's L1
+// and non-coherent L2 lines) before the workgroup reads what others wrote.
+// Returns false when the batch was aborted.
 __device__ bool prob_barrier(BatchState* st, int nb, int* abort_flag, uint64_t limit_ticks) {
-  __threadfence();
   __syncthreads();
   __shared__ int s_ok;
   if (threadIdx.x == 0) {
+    __threadfence();
     const unsigned g = __hip_atomic_load(&st->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned arrived =
         __hip_atomic_fetch_add(&st->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
@@ -84,10 +88,10 @@ __device__ bool prob_barrier(BatchState* st, int nb, int* abort_flag, uint64_t l
       }
     }
     if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
+    __threadfence();
     s_ok = ok;
   }
   __syncthreads();
-  __threadfence();
   return s_ok != 0;
 }
 
@@ -251,6 +255,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(
   double* App = Ap + P.row0;
   double* part = partials + P.part0;
   const int ntiles = (int)cdiv((int64_t)P.nslices * kWave, kBlock);
+  if (i == 0 && t == 0) st->t_begin = wall_clock64();
 
   // 1. element matrices and source vectors (as element_kernel, k = 1)
   {
@@ -353,6 +358,7 @@ __global__ __launch_bounds__(kBlock) void batch_kernel(
     normr = sqrt(rtrans);
   }
   if (i == 0 && t == 0) {
+    st->t_end = wall_clock64();
     st->iters = k - 1;
     st->status = status;
     st->normr = normr;
@@ -367,9 +373,13 @@ using namespace mfe;
 
 struct mfe_batch {
   mfe_ctx* ctx = nullptr;
-  int nprob = 0, grid = 0;
+  int nprob = 0, grid = 0, capacity = 0;
+  double tick_seconds = 1e-8;  // wall clock period
+  std::vector<double> last_seconds;  // per problem, previous solve
   std::vector<BatchProb> probs;
-  std::vector<int> soff;  // all problems' slice offsets
+  std::vector<int> soff;      // all problems' slice offsets
+  std::vector<int> blk2prob;  // problem of each workgroup
+  std::vector<int> last_iters;  // iterations of the previous solve (rebalancing)
   BatchProb* d_probs = nullptr;
   int* d_blk2prob = nullptr;
   int* d_soff = nullptr;
@@ -401,6 +411,51 @@ struct mfe_batch {
 
 static int host_axis_len(int i, int g) { return 1 + (i > 0) + (i < g - 1); }
 
+// Workgroups per problem: one each, then the rest one at a time to the
+// problem with the most work per workgroup (capped at its tile count).
+static void batch_assign(mfe_batch* B, const std::vector<double>& work, int grid) {
+  const int nprob = B->nprob;
+  std::vector<int> nb((size_t)nprob, 1);
+  int left = grid - nprob;
+  while (left > 0) {
+    int best = -1;
+    double bw = -1;
+    for (int q = 0; q < nprob; ++q) {
+      const int tiles = (int)cdiv((int64_t)B->probs[(size_t)q].nslices * kWave, kBlock);
+      if (nb[(size_t)q] >= tiles) continue;
+      const double w = work[(size_t)q] / nb[(size_t)q];
+      if (w > bw) {
+        bw = w;
+        best = q;
+      }
+    }
+    if (best < 0) break;
+    ++nb[(size_t)best];
+    --left;
+  }
+  B->blk2prob.clear();
+  int64_t part0 = 0;
+  for (int q = 0; q < nprob; ++q) {
+    BatchProb& P = B->probs[(size_t)q];
+    P.b0 = (int)B->blk2prob.size();
+    P.nb = nb[(size_t)q];
+    P.part0 = part0;
+    part0 += 2 * (int64_t)P.nb;
+    for (int k = 0; k < P.nb; ++k) B->blk2prob.push_back(q);
+  }
+  B->grid = (int)B->blk2prob.size();
+}
+
+static int batch_upload_assignment(mfe_ctx* ctx, mfe_batch* B) {
+  hipStream_t s = ctx->stream;
+  MFE_CHECK(ctx, hipMemcpyAsync(B->d_probs, B->probs.data(), sizeof(BatchProb) * B->probs.size(),
+                                hipMemcpyHostToDevice, s));
+  MFE_CHECK(ctx, hipMemcpyAsync(B->d_blk2prob, B->blk2prob.data(),
+                                sizeof(int) * B->blk2prob.size(), hipMemcpyHostToDevice, s));
+  MFE_CHECK(ctx, hipStreamSynchronize(s));
+  return MFE_OK;
+}
+
 extern "C" {
 
 int mfe_batch_create(mfe_ctx* ctx, int nprob, const int* dims, mfe_batch** out) {
@@ -422,6 +477,7 @@ int mfe_batch_create(mfe_ctx* ctx, int nprob, const int* dims, mfe_batch** out) 
   B->ctx = ctx;
   B->nprob = nprob;
   B->grid = grid;
+  B->capacity = grid;
   // Problem layout and slice widths (closed form: a row's length is the
   // product of its per-axis neighbour counts).
   std::vector<double> work((size_t)nprob);
@@ -456,43 +512,14 @@ int mfe_batch_create(mfe_ctx* ctx, int nprob, const int* dims, mfe_batch** out) 
     soff0 += P.nslices + 1;
     slot0 += (int64_t)acc * kWave + kSellTail;  // each problem keeps its own over-read tail
     elem0 += 44 * (int64_t)P.ne;
+    // CG iterations grow with the largest side (the smallest mesh width)
     const double nnz = (3.0 * gx - 2) * (3.0 * gy - 2) * (3.0 * gz - 2);
-    work[(size_t)q] = nnz * (double)(P.Nx + P.Ny + P.Nz);
+    work[(size_t)q] = nnz * (double)std::max(P.Nx, std::max(P.Ny, P.Nz));
     B->probs.push_back(P);
   }
   B->rows = row0;
   B->slots = slot0;
-  // Workgroups per problem: one each, then the rest one at a time to the
-  // problem with the most work per workgroup (capped at its tile count).
-  std::vector<int> nb((size_t)nprob, 1);
-  int left = grid - nprob;
-  while (left > 0) {
-    int best = -1;
-    double bw = -1;
-    for (int q = 0; q < nprob; ++q) {
-      const int tiles = (int)cdiv((int64_t)B->probs[(size_t)q].nslices * kWave, kBlock);
-      if (nb[(size_t)q] >= tiles) continue;
-      const double w = work[(size_t)q] / nb[(size_t)q];
-      if (w > bw) {
-        bw = w;
-        best = q;
-      }
-    }
-    if (best < 0) break;
-    ++nb[(size_t)best];
-    --left;
-  }
-  std::vector<int> blk2prob;
-  int64_t part0 = 0;
-  for (int q = 0; q < nprob; ++q) {
-    BatchProb& P = B->probs[(size_t)q];
-    P.b0 = (int)blk2prob.size();
-    P.nb = nb[(size_t)q];
-    P.part0 = part0;
-    part0 += 2 * (int64_t)P.nb;
-    for (int k = 0; k < P.nb; ++k) blk2prob.push_back(q);
-  }
-  B->grid = (int)blk2prob.size();
+  batch_assign(B, work, grid);
   hipStream_t s = ctx->stream;
   auto fail_oom = [&](hipError_t e) {
     B->release();
@@ -503,7 +530,7 @@ int mfe_batch_create(mfe_ctx* ctx, int nprob, const int* dims, mfe_batch** out) 
 #define BALLOC(ptr, bytes) \
   if ((e = hipMalloc(&(ptr), (bytes))) != hipSuccess) return fail_oom(e)
   BALLOC(B->d_probs, sizeof(BatchProb) * (size_t)nprob);
-  BALLOC(B->d_blk2prob, sizeof(int) * blk2prob.size());
+  BALLOC(B->d_blk2prob, sizeof(int) * (size_t)B->capacity);  // any later re-division fits
   BALLOC(B->d_soff, sizeof(int) * B->soff.size());
   BALLOC(B->d_scols, sizeof(int) * (size_t)B->slots);
   BALLOC(B->d_svals, sizeof(double) * (size_t)B->slots);
@@ -513,17 +540,14 @@ int mfe_batch_create(mfe_ctx* ctx, int nprob, const int* dims, mfe_batch** out) 
   BALLOC(B->d_r, sizeof(double) * (size_t)B->rows);
   BALLOC(B->d_p, sizeof(double) * (size_t)B->rows);
   BALLOC(B->d_Ap, sizeof(double) * (size_t)B->rows);
-  BALLOC(B->d_part, sizeof(double) * (size_t)part0);
+  BALLOC(B->d_part, sizeof(double) * 2 * (size_t)B->capacity);
   BALLOC(B->d_states, sizeof(BatchState) * (size_t)nprob);
   BALLOC(B->d_abort, sizeof(int));
 #undef BALLOC
   // zero everything once: the per-problem tails stay value 0, column 0
   MFE_CHECK(ctx, hipMemsetAsync(B->d_scols, 0, sizeof(int) * (size_t)B->slots, s));
   MFE_CHECK(ctx, hipMemsetAsync(B->d_svals, 0, sizeof(double) * (size_t)B->slots, s));
-  MFE_CHECK(ctx, hipMemcpyAsync(B->d_probs, B->probs.data(), sizeof(BatchProb) * (size_t)nprob,
-                                hipMemcpyHostToDevice, s));
-  MFE_CHECK(ctx, hipMemcpyAsync(B->d_blk2prob, blk2prob.data(), sizeof(int) * blk2prob.size(),
-                                hipMemcpyHostToDevice, s));
+  if (int st2 = batch_upload_assignment(ctx, B)) return st2;
   MFE_CHECK(ctx, hipMemcpyAsync(B->d_soff, B->soff.data(), sizeof(int) * B->soff.size(),
                                 hipMemcpyHostToDevice, s));
   MFE_CHECK(ctx, hipStreamSynchronize(s));
@@ -571,7 +595,22 @@ int mfe_batch_solve(mfe_ctx* ctx, mfe_batch* B, int max_iter, double rel_tol, in
   MFE_CHECK(ctx, hipMemsetAsync(B->d_abort, 0, sizeof(int), s));
   int rate_khz = 0;
   MFE_CHECK(ctx, hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, ctx->device));
-  uint64_t limit = (uint64_t)(kBatchWaitSeconds * 1e3 * (double)(rate_khz > 0 ? rate_khz : 100000));
+  if (rate_khz <= 0) rate_khz = 100000;
+  B->tick_seconds = 1.0 / (1e3 * (double)rate_khz);
+  uint64_t limit = (uint64_t)(kBatchWaitSeconds * 1e3 * (double)rate_khz);
+  // Autotuned balance: after a solve, give each problem workgroups in
+  // proportion to the traffic its measured iterations moved.
+  if ((int)B->last_iters.size() == B->nprob) {
+    std::vector<double> work((size_t)B->nprob);
+    for (int q = 0; q < B->nprob; ++q) {
+      const BatchProb& P = B->probs[(size_t)q];
+      const double stored = (double)B->soff[(size_t)(P.soff0 + P.nslices)] * kWave;
+      work[(size_t)q] = (double)B->last_iters[(size_t)q] * (12.0 * stored + 88.0 * P.n) +
+                        400.0 * P.ne;
+    }
+    batch_assign(B, work, B->capacity);
+    if (int st2 = batch_upload_assignment(ctx, B)) return st2;
+  }
   HexTables tab;
   host_tables(tab);
   void* args[] = {&B->d_probs, &B->d_blk2prob, &tab,        &B->d_soff,   &B->d_scols,
@@ -595,7 +634,12 @@ int mfe_batch_solve(mfe_ctx* ctx, mfe_batch* B, int max_iter, double rel_tol, in
     return fail(ctx, MFE_HIP_ERROR, "batch_solve: a problem barrier waited past its limit; "
                                     "the launch was abandoned");
   int status = MFE_OK;
+  B->last_iters.assign((size_t)B->nprob, 0);
+  B->last_seconds.assign((size_t)B->nprob, 0.0);
   for (int q = 0; q < B->nprob; ++q) {
+    B->last_iters[(size_t)q] = hs[(size_t)q].iters;
+    B->last_seconds[(size_t)q] =
+        (double)(hs[(size_t)q].t_end - hs[(size_t)q].t_begin) * B->tick_seconds;
     if (iters) iters[q] = hs[(size_t)q].iters;
     if (rel_residual)
       rel_residual[q] = hs[(size_t)q].bnorm > 0 ? hs[(size_t)q].normr / hs[(size_t)q].bnorm
@@ -604,6 +648,13 @@ int mfe_batch_solve(mfe_ctx* ctx, mfe_batch* B, int max_iter, double rel_tol, in
   }
   if (status == MFE_BREAKDOWN) fail(ctx, MFE_BREAKDOWN, "batch_solve: CG breakdown in a problem");
   return status;
+}
+
+int mfe_batch_timing(const mfe_batch* B, double* seconds) {
+  if (!B || !seconds) return MFE_BAD_ARG;
+  for (int q = 0; q < B->nprob; ++q)
+    seconds[q] = (size_t)q < B->last_seconds.size() ? B->last_seconds[(size_t)q] : 0.0;
+  return MFE_OK;
 }
 
 int mfe_batch_download(mfe_ctx* ctx, const mfe_batch* B, int p, int* row_offsets, int* cols,

@@ -572,6 +572,7 @@ __global__ __launch_bounds__(kBlock) void dot_seq_kernel(int64_t n, const double
 struct CgState {
   double rtrans, oldrtrans, pAp, alpha, beta, normr, brkdown_tol, tol;
   double alpha_prev;  // alpha of the iteration whose x update is pending
+  double gamma;       // coefficient of r in the next p update (1 but after a resume)
   int k, max_iter, done, status, num_iters, need_norms, x_pending, pad;
 };
 
@@ -611,6 +612,7 @@ __device__ double* hist_row(const CgState* st, double* hist) {
 // (cg_solve.hpp:186-201)
 __device__ void cg_alpha_or_defer(CgState* st, double pAp, double* hist) {
   st->x_pending = 0;  // cg_p of this iteration has applied it
+  st->gamma = 1.0;
   st->pAp = pAp;
   double* h = hist_row(st, hist);
   h[0] = st->rtrans;
@@ -650,6 +652,7 @@ __global__ void cg_init_kernel(CgState* st, int max_iter, double tol) {
   s.max_iter = max_iter;
   s.tol = tol;
   s.brkdown_tol = DBL_EPSILON;
+  s.gamma = 1.0;
   s.k = 1;
   *st = s;
 }
@@ -708,18 +711,24 @@ __global__ __launch_bounds__(kBlock) void cg_next_fast_kernel(CgState* st, const
 }
 
 // Continue a stopped CG after its residual was replaced (the mixed solve's
-// FP64 residual, rounded): rtrans = r.r of the new residual, beta = rtrans /
-// rtrans of the last iteration, so the next cg_p forms p = r + beta p with
-// the kept direction.  k restarts at 2 (history rows stay in range).
+// FP64 residual, rounded): rtrans = r.r of the new residual and beta = rtrans
+// / rtrans of the last iteration.  The replaced r is no longer orthogonal to
+// the kept p, so p first loses its r component, p <- p - (r.p / r.r) r (the
+// "reliable update" of mixed-precision CG, Clark et al. 2010); folded into
+// the next cg_p as p = gamma r + beta p with gamma = 1 - beta (r.p / r.r).
+// k restarts at 1 (history rows stay in range; the captured iterations always
+// take p = gamma r + beta p).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void cg_resume_kernel(CgState* st, const T* partials, int np,
                                                            int max_iter, double tol) {
   __shared__ T red[4];
   const T rr = sum_partials<T>(partials, np, red);
+  const T rp = sum_partials<T>(partials + np, np, red);
   if (threadIdx.x == 0) {
     st->oldrtrans = st->rtrans;
     st->rtrans = (double)rr;
     st->beta = st->oldrtrans > 0 ? st->rtrans / st->oldrtrans : 0.0;
+    st->gamma = st->rtrans > 0 ? 1.0 - st->beta * ((double)rp / st->rtrans) : 1.0;
     st->normr = sqrt(st->rtrans);
     st->k = 1;
     st->max_iter = max_iter;
@@ -794,6 +803,7 @@ __global__ __launch_bounds__(kBlock) void cg_p_kernel(int64_t n, const CgState* 
                                                       const T* y, T* p, T* x) {
   if (st->done) return;
   const T beta = (T)st->beta;
+  const T gamma = (T)st->gamma;  // 1 (the reference's waxpby) except after a resume
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (st->x_pending) {
@@ -801,10 +811,10 @@ __global__ __launch_bounds__(kBlock) void cg_p_kernel(int64_t n, const CgState* 
     for (int64_t i = i0; i < n; i += stride) {
       const T pi = p[i];
       x[i] = T(1) * x[i] + a * pi;
-      p[i] = T(1) * r[i] + beta * (y == p ? pi : y[i]);
+      p[i] = gamma * r[i] + beta * (y == p ? pi : y[i]);
     }
   } else {
-    for (int64_t i = i0; i < n; i += stride) p[i] = T(1) * r[i] + beta * y[i];
+    for (int64_t i = i0; i < n; i += stride) p[i] = gamma * r[i] + beta * y[i];
   }
 }
 
@@ -1147,6 +1157,7 @@ static int run_cg_resume(mfe_ctx* ctx, const SpmvOp<T>& op, T* x, int max_iter, 
   if (n == 0) return MFE_OK;
   const unsigned vg = vec_grid(n);
   dot_partial_kernel<T><<<vg, kBlock, 0, s>>>(n, v.r, v.r, v.partials);
+  dot_partial_kernel<T><<<vg, kBlock, 0, s>>>(n, v.r, v.p, v.partials + vg);
   cg_resume_kernel<T><<<1, kBlock, 0, s>>>(w->st, v.partials, (int)vg, max_iter, tol);
   MFE_LAUNCHED(ctx, "cg_resume_kernel");
   const bool reuse = slot.exec && slot.kA == op.A && slot.kgen == op.gen &&

@@ -86,11 +86,15 @@ def test_batch_64_boxes(mfe, oracle_mod):
     B = mfe.Batch(dims)
     inf = B.info()
     say(f"  info {inf}")
-    B.solve(max_iter=5000, rel_tol=1e-10)  # warm
-    out = B.solve(max_iter=5000, rel_tol=1e-10)
+    first = B.solve(max_iter=5000, rel_tol=1e-10)  # model-balanced
+    out = B.solve(max_iter=5000, rel_tol=1e-10)  # balanced by the first solve's iterations
     assert out["status"] == mfe.MFE_OK
     assert np.all(out["rel_residual"] <= 1e-10)
-    say(f"  {out['seconds']:.3f} s, iters {out['iters'].min()}..{out['iters'].max()}, "
+    assert np.array_equal(first["iters"], out["iters"])
+    ps = out["problem_seconds"]
+    say(f"  first {first['seconds']:.3f} s (problems {first['problem_seconds'].min():.3f}.."
+        f"{first['problem_seconds'].max():.3f}), rebalanced {out['seconds']:.3f} s "
+        f"({ps.min():.3f}..{ps.max():.3f}); iters {out['iters'].min()}..{out['iters'].max()}, "
         f"sum {int(out['iters'].sum())}")
     vol = dims.prod(axis=1)
     picks = {int(np.argmin(vol)), int(np.argmax(vol)), 7}
