@@ -90,7 +90,8 @@ def test_batch_64_boxes(mfe, oracle_mod):
     out = B.solve(max_iter=5000, rel_tol=1e-10)  # balanced by the first solve's iterations
     assert out["status"] == mfe.MFE_OK
     assert np.all(out["rel_residual"] <= 1e-10)
-    assert np.array_equal(first["iters"], out["iters"])
+    # a new division changes the dots' summation order, hence the last bits
+    assert np.max(np.abs(first["iters"].astype(int) - out["iters"].astype(int))) <= 2
     ps = out["problem_seconds"]
     say(f"  first {first['seconds']:.3f} s (problems {first['problem_seconds'].min():.3f}.."
         f"{first['problem_seconds'].max():.3f}), rebalanced {out['seconds']:.3f} s "
