@@ -301,7 +301,7 @@ def main():
             xk.fill(0.0)
             ctx.synchronize()
             t0 = time.perf_counter()
-            rk = mfe.cg_solve_mixed(A, b, xk, inner_tol=1e-5, outer_tol=1e-10,
+            rk = mfe.cg_solve_mixed(A, b, xk, inner_tol=0.1, outer_tol=1e-10,
                                     keep_direction=True)
             tk = time.perf_counter() - t0
         xkh = xk.to_numpy()
@@ -321,7 +321,7 @@ def main():
                    "mixed_true_rel_residual": relm, "mixed_converged": rm.converged,
                    "mixed_rel_err_vs_fp64": err, "speedup_mixed_vs_fp64": tt / tm,
                    "mixed_keep_direction": {
-                       "s": tk, "outer": rk.outer_iters, "inner": rk.inner_iters,
+                       "inner_tol": 0.1, "s": tk, "outer": rk.outer_iters, "inner": rk.inner_iters,
                        "true_rel_residual": relk, "converged": rk.converged,
                        "rel_err_vs_fp64": errk, "speedup_vs_fp64": tt / tk}}
         log(f"time to 1e-10: fp64 {tt:.3f}s ({r2.num_iters} its, true rel {rel64:.2e}); mixed "

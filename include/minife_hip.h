@@ -208,8 +208,11 @@ int mfe_cg_solve_mixed(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec
 int mfe_matvec_fp32(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y);
 /* mfe_cg_solve_mixed with flags.  MFE_MIXED_KEEP_DIRECTION: the inner FP32 CG
  * is not restarted at a refinement; its residual is replaced by the rounded
- * FP64 residual b - A x and it continues with its search direction (reliable
- * updates), so the inner iterations total about what FP64 CG needs. */
+ * FP64 residual b - A x, its direction loses its component along the new
+ * residual, and it continues (reliable updates, Clark et al. 2010), so the
+ * inner iterations total about what FP64 CG needs.  It wants frequent
+ * replacements: inner_tol (the reduction per refinement) about 0.1; at 1e-5
+ * the FP32 recursion drifts too far from the true residual between two. */
 #define MFE_MIXED_KEEP_DIRECTION 0x1u
 int mfe_cg_solve_mixed_ex(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_vec* x,
                           int max_outer, int max_inner, double inner_tol, double outer_tol,

@@ -92,6 +92,19 @@ def _share_torch_hip_runtime() -> None:
             return
 
 
+def _release(obj) -> None:
+    """Destructor body: free the handle unless the interpreter is shutting
+    down (the HIP runtime may already be torn down; the process exit returns
+    the device memory anyway)."""
+    import sys
+    if sys.is_finalizing():
+        return
+    try:
+        obj.close()
+    except Exception:
+        pass
+
+
 def lib() -> ctypes.CDLL:
     """Load libminife_hip.so (raises if it has not been built)."""
     global _lib
@@ -220,10 +233,7 @@ class Context:
             self.h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _release(self)
 
 
 _default_ctx: Context | None = None
@@ -291,10 +301,7 @@ class CSRMatrix:
             self.h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _release(self)
 
 
 class Vector:
@@ -342,10 +349,7 @@ class Vector:
             self.h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _release(self)
 
 
 def _box6(box) -> "ctypes.Array":
@@ -517,13 +521,17 @@ class MixedResult:
 MIXED_KEEP_DIRECTION = 0x1
 
 
-def cg_solve_mixed(A: CSRMatrix, b: Vector, x: Vector, *, max_outer: int = 20,
-                   max_inner: int = 2000, inner_tol: float = 1e-5,
+def cg_solve_mixed(A: CSRMatrix, b: Vector, x: Vector, *, max_outer: int = 200,
+                   max_inner: int = 20000, inner_tol: float | None = None,
                    outer_tol: float = 1e-10, keep_direction: bool = False) -> MixedResult:
     """Mixed-precision iterative refinement around cg_solve (FP32 inner CG on an
-    FP32 copy of A, FP64 residual/update/norms on the GPU).  keep_direction:
-    the inner CG continues across refinements with its residual replaced by
-    the FP64 one (reliable updates) instead of restarting."""
+    FP32 copy of A, FP64 residual/update/norms on the GPU).  inner_tol is the
+    residual reduction per refinement (default 1e-5 restarting, 0.1 keeping
+    the direction).  keep_direction: the inner CG continues across
+    refinements with its residual replaced by the FP64 one (reliable updates)
+    instead of restarting; it needs frequent replacements (0.1)."""
+    if inner_tol is None:
+        inner_tol = 0.1 if keep_direction else 1e-5
     oi, ii = ctypes.c_int(0), ctypes.c_int(0)
     rel = ctypes.c_double(0)
     times = np.zeros(5, np.float64)
@@ -654,10 +662,7 @@ class Batch:
             self.h = None
 
     def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
+        _release(self)
 
 
 def soln(x: float, y: float, z: float, max_p: int = 300, max_q: int = 300) -> float:
