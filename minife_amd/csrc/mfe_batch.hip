@@ -24,6 +24,7 @@
 //      take the same scalars and leave the loop at the same iteration.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "mfe_common.hpp"
@@ -234,7 +235,11 @@ __device__ __forceinline__ double batch_row_sum(const BatchProb& P, int row, con
   return sell_row_sum<double>(scols, svals, pv, P.slot0 + (int64_t)o * kWave, w, lane, len, 0.0);
 }
 
-__global__ __launch_bounds__(kBlock) void batch_kernel(
+// WAVES: the minimum waves per SIMD the register allocation must allow (the
+// element math alone wants ~200 VGPRs, i.e. 2 waves; more waves hide more of
+// the CG's memory latency at the price of some spilling in the element phase).
+template <int WAVES>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void batch_kernel(
     const BatchProb* __restrict__ probs, const int* __restrict__ blk2prob, HexTables tab,
     const int* __restrict__ soff_all, int* scols, double* svals, double* KF, double* b,
     double* x, double* r, double* p, double* Ap, double* partials, BatchState* states,
@@ -374,6 +379,7 @@ using namespace mfe;
 struct mfe_batch {
   mfe_ctx* ctx = nullptr;
   int nprob = 0, grid = 0, capacity = 0;
+  const void* kernel = nullptr;  // the batch_kernel instance (occupancy) in use
   double tick_seconds = 1e-8;  // wall clock period
   std::vector<double> last_seconds;  // per problem, previous solve
   std::vector<BatchProb> probs;
@@ -465,8 +471,13 @@ int mfe_batch_create(mfe_ctx* ctx, int nprob, const int* dims, mfe_batch** out) 
     if (dims[q] < 1 || dims[q] > 1024)
       return fail(ctx, MFE_BAD_ARG, "batch_create: box sides must be in [1, 1024] elements");
   MFE_CHECK(ctx, hipSetDevice(ctx->device));
+  int waves = 2;
+  if (const char* e = std::getenv("MINIFE_BATCH_WAVES")) waves = std::atoi(e);
+  if (waves < 2 || waves > 4) waves = 2;
+  const void* kern = waves == 4 ? (const void*)batch_kernel<4>
+                                : waves == 3 ? (const void*)batch_kernel<3> : (const void*)batch_kernel<2>;
   int per_cu = 0;
-  MFE_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel, kBlock, 0));
+  MFE_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
   int coop = 0;
   MFE_CHECK(ctx, hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, ctx->device));
   const int grid = per_cu * ctx->cu_count;
@@ -478,6 +489,7 @@ int mfe_batch_create(mfe_ctx* ctx, int nprob, const int* dims, mfe_batch** out) 
   B->nprob = nprob;
   B->grid = grid;
   B->capacity = grid;
+  B->kernel = kern;
   // Problem layout and slice widths (closed form: a row's length is the
   // product of its per-axis neighbour counts).
   std::vector<double> work((size_t)nprob);
@@ -618,7 +630,7 @@ int mfe_batch_solve(mfe_ctx* ctx, mfe_batch* B, int max_iter, double rel_tol, in
                   &B->d_p,     &B->d_Ap,       &B->d_part,  &B->d_states, &B->d_abort,
                   &max_iter,   &rel_tol,       &limit};
   MFE_CHECK(ctx, hipEventRecord(ctx->t_start, s));
-  MFE_CHECK(ctx, hipLaunchCooperativeKernel((const void*)batch_kernel, dim3((unsigned)B->grid),
+  MFE_CHECK(ctx, hipLaunchCooperativeKernel(B->kernel, dim3((unsigned)B->grid),
                                             dim3(kBlock), args, 0, s));
   MFE_CHECK(ctx, hipEventRecord(ctx->t_stop, s));
   MFE_CHECK(ctx, hipEventSynchronize(ctx->t_stop));
