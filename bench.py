@@ -100,12 +100,18 @@ def bench_batched(mfe, ctx):
         inf = B.info(p)
         byts += int(out["iters"][p]) * (12 * inf["stored"] + 88 * inf["rows"])
     tot = B.info()
+    slow = int(np.argmax(out["problem_seconds"]))
+    slowest = {"problem": slow, "dims": [int(v) for v in B.dims[slow]],
+               "iters": int(out["iters"][slow]), "workgroups": B.info(slow)["workgroups"],
+               "seconds": float(out["problem_seconds"][slow]),
+               "phase_seconds": {k: round(float(v), 5)
+                                 for k, v in zip(B.PHASES, out["phase_seconds"][slow])}}
     res = {"workload": "64 boxes, sides U[16,96] seed 0, assemble + FP64 CG to 1e-10, one launch",
            "seconds": out["seconds"], "problems": B.nprob, "workgroups": tot["workgroups"],
            "rows": tot["rows"], "nnz": tot["nnz"], "iters_total": int(out["iters"].sum()),
            "iters_max": int(out["iters"].max()),
            "max_rel_residual": float(out["rel_residual"].max()),
-           "cg_equiv_GBs": byts / out["seconds"] / 1e9}
+           "cg_equiv_GBs": byts / out["seconds"] / 1e9, "slowest_problem": slowest}
     B.close()
     return res
 

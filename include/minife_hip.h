@@ -253,10 +253,12 @@ int mfe_batch_info(const mfe_batch* batch, int p, int64_t info[6]);
 int mfe_batch_solve(mfe_ctx* ctx, mfe_batch* batch, int max_iter, double rel_tol, int* iters,
                     double* rel_residual, double* seconds);
 /* seconds[nprob]: each problem's time in the last solve (its first
- * workgroup's wall clock from entry to exit).  Every solve after the first
- * re-divides the workgroups in proportion to the traffic the previous solve's
- * iteration counts moved (the batch tunes its own balance). */
-int mfe_batch_timing(const mfe_batch* batch, double* seconds);
+ * workgroup's wall clock from entry to exit); phases[6*nprob] (nullable):
+ * that workgroup's seconds in element math, row assembly, p updates, SpMVs,
+ * x/r updates and barrier waits.  Every solve after the first re-divides the
+ * workgroups in proportion to the traffic the previous solve's iteration
+ * counts moved (the batch tunes its own balance). */
+int mfe_batch_timing(const mfe_batch* batch, double* seconds, double* phases);
 /* Problem p's assembled operator as CSR (row_offsets[n+1], cols[nnz],
  * vals[nnz]), its right-hand side b[n] and solution x[n]; any may be NULL. */
 int mfe_batch_download(mfe_ctx* ctx, const mfe_batch* batch, int p, int* row_offsets, int* cols,

@@ -177,7 +177,7 @@ def lib() -> ctypes.CDLL:
         "mfe_batch_solve": ([_vp, _vp, ctypes.c_int, ctypes.c_double, _vp, _vp,
                              ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "mfe_batch_download": ([_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp], ctypes.c_int),
-        "mfe_batch_timing": ([_vp, _vp], ctypes.c_int),
+        "mfe_batch_timing": ([_vp, _vp, _vp], ctypes.c_int),
         "mfe_verify_solution": ([_vp, _int6, _vp, ctypes.c_double, ctypes.c_int,
                                  ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     }
@@ -640,9 +640,12 @@ class Batch:
         if st not in (MFE_OK, MFE_BREAKDOWN):
             self.ctx.check(st, "batch_solve")
         per = np.zeros(self.nprob, np.float64)
-        lib().mfe_batch_timing(self.h, _ptr(per))
+        ph = np.zeros((self.nprob, 6), np.float64)
+        lib().mfe_batch_timing(self.h, _ptr(per), _ptr(ph))
         return dict(iters=iters, rel_residual=rel, seconds=sec.value, status=st,
-                    problem_seconds=per)
+                    problem_seconds=per, phase_seconds=ph)
+
+    PHASES = ("elements", "rows", "p_update", "spmv", "xr_update", "barrier_wait")
 
     def download(self, p: int, matrix: bool = True) -> dict:
         inf = self.info(p)

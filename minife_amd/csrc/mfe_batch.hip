@@ -51,9 +51,13 @@ struct BatchState {  // per problem, zeroed before each launch
   int iters, status;  // status: 0 converged or max_iter, 1 breakdown, 2 aborted
   double normr, tol, bnorm, pad;
   uint64_t t_begin, t_end;  // wall clock of the problem's first workgroup
+  // that workgroup's clock ticks per phase: element math, row assembly,
+  // p update, SpMV, x/r update, barrier waits
+  uint64_t t_phase[6];
 };
 
 constexpr double kBatchWaitSeconds = 10.0;
+constexpr int kUnroll = 4;  // tiles per step of the vector loops
 
 // Problem-local barrier.  The workgroup's stores are complete in L2 after
 // __syncthreads (L1 writes through); thread 0 then releases them at agent
@@ -260,7 +264,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
   double* App = Ap + P.row0;
   double* part = partials + P.part0;
   const int ntiles = (int)cdiv((int64_t)P.nslices * kWave, kBlock);
-  if (i == 0 && t == 0) st->t_begin = wall_clock64();
+  const bool timer = i == 0 && t == 0;
+  uint64_t tp[6] = {0, 0, 0, 0, 0, 0};
+  uint64_t tc = wall_clock64();
+  if (timer) st->t_begin = tc;
+#define MFE_PHASE(k)                  \
+  do {                                \
+    const uint64_t now_ = wall_clock64(); \
+    tp[k] += now_ - tc;               \
+    tc = now_;                        \
+  } while (0)
+#define MFE_BARRIER()                                                  \
+  do {                                                                 \
+    const bool ok_ = prob_barrier(st, nb, abort_flag, limit_ticks);    \
+    MFE_PHASE(5);                                                      \
+    if (!ok_) return;                                                  \
+  } while (0)
 
   // 1. element matrices and source vectors (as element_kernel, k = 1)
   {
@@ -276,14 +295,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
       const double c[24] = {x0, y0, z0, x1, y0, z0, x1, y1, z0, x0, y1, z0,
                             x0, y0, z1, x1, y0, z1, x1, y1, z1, x0, y1, z1};
       double Ke[36], Fe[8];
-      hex8_element(c, tab, 1.0, Ke, Fe);
+      hex8_element<WAVES >= 3>(c, tab, 1.0, Ke, Fe);
 #pragma unroll
       for (int q = 0; q < 36; ++q) K[(int64_t)q * ne + e] = Ke[q];
 #pragma unroll
       for (int q = 0; q < 8; ++q) F[(int64_t)q * ne + e] = Fe[q];
     }
   }
-  if (!prob_barrier(st, nb, abort_flag, limit_ticks)) return;
+  MFE_PHASE(0);
+  MFE_BARRIER();
 
   // 2. rows: gather, Dirichlet, SELL store; x = 0, r = b
   double acc = 0.0;
@@ -303,7 +323,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
     const double v0 = block_sum<double>(acc, red);
     if (t == 0) part[i] = v0;
   }
-  if (!prob_barrier(st, nb, abort_flag, limit_ticks)) return;
+  MFE_PHASE(1);
+  MFE_BARRIER();
   double rtrans = sum_partials_all(part, nb, red);
   int buf = 1;
   const double bnorm = sqrt(rtrans);
@@ -314,11 +335,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
   // 3. CG
   for (; k <= max_iter && normr > tol; ++k) {
     const double beta = k == 1 ? 0.0 : rtrans / oldrtrans;
-    for (int tile = i; tile < ntiles; tile += nb) {
-      const int row = tile * kBlock + t;
-      if (row < P.n) pp[row] = k == 1 ? 1.0 * rp[row] + 0.0 * rp[row] : 1.0 * rp[row] + beta * pp[row];
+    // four of the workgroup's tiles per step, loads before stores
+    for (int tile = i; tile < ntiles; tile += kUnroll * nb) {
+      double rv[kUnroll], pv[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int row = (tile + u * nb) * kBlock + t;
+        const bool ok = tile + u * nb < ntiles && row < P.n;
+        rv[u] = ok ? rp[row] : 0.0;
+        pv[u] = ok && k > 1 ? pp[row] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int row = (tile + u * nb) * kBlock + t;
+        if (tile + u * nb < ntiles && row < P.n)
+          pp[row] = k == 1 ? 1.0 * rv[u] + 0.0 * rv[u] : 1.0 * rv[u] + beta * pv[u];
+      }
     }
-    if (!prob_barrier(st, nb, abort_flag, limit_ticks)) return;
+    MFE_PHASE(2);
+    MFE_BARRIER();
     acc = 0.0;
     for (int tile = i; tile < ntiles; tile += nb) {
       const int row = tile * kBlock + t;
@@ -334,7 +369,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
       const double v0 = block_sum<double>(acc, red);
       if (t == 0) part[buf * nb + i] = v0;
     }
-    if (!prob_barrier(st, nb, abort_flag, limit_ticks)) return;
+    MFE_PHASE(3);
+    MFE_BARRIER();
     const double pAp = sum_partials_all(part + buf * nb, nb, red);
     buf ^= 1;
     if (!(pAp > 0.0)) {  // non-positive curvature: the operator is not SPD here
@@ -343,26 +379,43 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
     }
     const double alpha = rtrans / pAp;
     acc = 0.0;
-    for (int tile = i; tile < ntiles; tile += nb) {
-      const int row = tile * kBlock + t;
-      if (row < P.n) {
-        xp[row] = 1.0 * xp[row] + alpha * pp[row];
-        const double rv = 1.0 * rp[row] + (-alpha) * App[row];
-        rp[row] = rv;
-        acc += rv * rv;
+    for (int tile = i; tile < ntiles; tile += kUnroll * nb) {
+      double xv[kUnroll], pv[kUnroll], rv[kUnroll], av[kUnroll];
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int row = (tile + u * nb) * kBlock + t;
+        const bool ok = tile + u * nb < ntiles && row < P.n;
+        xv[u] = ok ? xp[row] : 0.0;
+        pv[u] = ok ? pp[row] : 0.0;
+        rv[u] = ok ? rp[row] : 0.0;
+        av[u] = ok ? App[row] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u) {
+        const int row = (tile + u * nb) * kBlock + t;
+        if (tile + u * nb < ntiles && row < P.n) {
+          xp[row] = 1.0 * xv[u] + alpha * pv[u];
+          const double r1 = 1.0 * rv[u] + (-alpha) * av[u];
+          rp[row] = r1;
+          acc += r1 * r1;
+        }
       }
     }
     {
       const double v0 = block_sum<double>(acc, red);
       if (t == 0) part[buf * nb + i] = v0;
     }
-    if (!prob_barrier(st, nb, abort_flag, limit_ticks)) return;
+    MFE_PHASE(4);
+    MFE_BARRIER();
     oldrtrans = rtrans;
     rtrans = sum_partials_all(part + buf * nb, nb, red);
     buf ^= 1;
     normr = sqrt(rtrans);
   }
-  if (i == 0 && t == 0) {
+#undef MFE_BARRIER
+#undef MFE_PHASE
+  if (timer) {
+    for (int q = 0; q < 6; ++q) st->t_phase[q] = tp[q];
     st->t_end = wall_clock64();
     st->iters = k - 1;
     st->status = status;
@@ -382,6 +435,7 @@ struct mfe_batch {
   const void* kernel = nullptr;  // the batch_kernel instance (occupancy) in use
   double tick_seconds = 1e-8;  // wall clock period
   std::vector<double> last_seconds;  // per problem, previous solve
+  std::vector<double> last_phases;   // per problem x 6 phases, previous solve
   std::vector<BatchProb> probs;
   std::vector<int> soff;      // all problems' slice offsets
   std::vector<int> blk2prob;  // problem of each workgroup
@@ -648,10 +702,13 @@ int mfe_batch_solve(mfe_ctx* ctx, mfe_batch* B, int max_iter, double rel_tol, in
   int status = MFE_OK;
   B->last_iters.assign((size_t)B->nprob, 0);
   B->last_seconds.assign((size_t)B->nprob, 0.0);
+  B->last_phases.assign((size_t)B->nprob * 6, 0.0);
   for (int q = 0; q < B->nprob; ++q) {
     B->last_iters[(size_t)q] = hs[(size_t)q].iters;
     B->last_seconds[(size_t)q] =
         (double)(hs[(size_t)q].t_end - hs[(size_t)q].t_begin) * B->tick_seconds;
+    for (int ph = 0; ph < 6; ++ph)
+      B->last_phases[(size_t)q * 6 + ph] = (double)hs[(size_t)q].t_phase[ph] * B->tick_seconds;
     if (iters) iters[q] = hs[(size_t)q].iters;
     if (rel_residual)
       rel_residual[q] = hs[(size_t)q].bnorm > 0 ? hs[(size_t)q].normr / hs[(size_t)q].bnorm
@@ -662,10 +719,15 @@ int mfe_batch_solve(mfe_ctx* ctx, mfe_batch* B, int max_iter, double rel_tol, in
   return status;
 }
 
-int mfe_batch_timing(const mfe_batch* B, double* seconds) {
-  if (!B || !seconds) return MFE_BAD_ARG;
-  for (int q = 0; q < B->nprob; ++q)
-    seconds[q] = (size_t)q < B->last_seconds.size() ? B->last_seconds[(size_t)q] : 0.0;
+int mfe_batch_timing(const mfe_batch* B, double* seconds, double* phases) {
+  if (!B) return MFE_BAD_ARG;
+  for (int q = 0; q < B->nprob; ++q) {
+    const bool have = (size_t)q < B->last_seconds.size();
+    if (seconds) seconds[q] = have ? B->last_seconds[(size_t)q] : 0.0;
+    if (phases)
+      for (int ph = 0; ph < 6; ++ph)
+        phases[q * 6 + ph] = have ? B->last_phases[(size_t)q * 6 + ph] : 0.0;
+  }
   return MFE_OK;
 }
 

@@ -72,6 +72,9 @@ __host__ __device__ constexpr int packed(int a, int b) {
 // Hex8::diffusionMatrix_symm (Hex8.hpp:259-361) and Hex8::sourceVector
 // (Hex8.hpp:363-410) for one element; J, detJ are evaluated once per Gauss
 // point and shared (the reference evaluates the same expressions twice).
+// LEAN keeps one Gauss point in flight (the loops are not unrolled), which
+// holds the register footprint near the 44 accumulators; same arithmetic.
+template <bool LEAN = false>
 __device__ __forceinline__ void hex8_element(const double c[24], const HexTables& tab, double k,
                                              double K[36], double F[8]) {
 #pragma unroll
@@ -79,14 +82,14 @@ __device__ __forceinline__ void hex8_element(const double c[24], const HexTables
 #pragma unroll
   for (int i = 0; i < 8; ++i) F[i] = 0;
   int q = 0;
-#pragma unroll
+#pragma unroll(LEAN ? 1 : 2)
   for (int ig = 0; ig < 2; ++ig) {
     const double wi = tab.gw[ig];
-#pragma unroll
+#pragma unroll(LEAN ? 1 : 2)
     for (int jg = 0; jg < 2; ++jg) {
       const double wj = tab.gw[jg];
       const double wi_wj = wi * wj;
-#pragma unroll
+#pragma unroll(LEAN ? 1 : 2)
       for (int kg = 0; kg < 2; ++kg, ++q) {
         const double wk = tab.gw[kg];
         const double wi_wj_wk = wi_wj * wk;
@@ -131,12 +134,41 @@ __device__ __forceinline__ void hex8_element(const double c[24], const HexTables
           for (int n = m + 1; n < 8; ++n)
             K[off++] += kdw * ((dx[m] * dx[n]) + (dy[m] * dy[n]) + (dz[m] * dz[n]));
         }
-        // sourceVector: term = Q*detJ*wi*wj*wk with Q = 1.0 (Hex8.hpp:379,402).
-        const double term = 1.0 * detJ * wi * wj * wk;
+        if (!LEAN) {
+          // sourceVector: term = Q*detJ*wi*wj*wk with Q = 1.0 (Hex8.hpp:379,402).
+          const double term = 1.0 * detJ * wi * wj * wk;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) F[i] += tab.psi[8 * q + i] * term;
+          for (int i = 0; i < 8; ++i) F[i] += tab.psi[8 * q + i] * term;
+        }
       }
     }
+  }
+  if (LEAN) {  // the source vector in a pass of its own: detJ from the same expressions
+    q = 0;
+#pragma unroll 1
+    for (int ig = 0; ig < 2; ++ig)
+#pragma unroll 1
+      for (int jg = 0; jg < 2; ++jg)
+#pragma unroll 1
+        for (int kg = 0; kg < 2; ++kg, ++q) {
+          const double* g = tab.gv + 24 * q;
+          double J00 = 0, J01 = 0, J02 = 0, J10 = 0, J11 = 0, J12 = 0, J20 = 0, J21 = 0, J22 = 0;
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            const double g0 = g[3 * i], g1 = g[3 * i + 1], g2 = g[3 * i + 2];
+            const double x = c[3 * i], y = c[3 * i + 1], z = c[3 * i + 2];
+            J00 += g0 * x; J01 += g0 * y; J02 += g0 * z;
+            J10 += g1 * x; J11 += g1 * y; J12 += g1 * z;
+            J20 += g2 * x; J21 += g2 * y; J22 += g2 * z;
+          }
+          const double term0 = J22 * J11 - J21 * J12;
+          const double term1 = J22 * J01 - J21 * J02;
+          const double term2 = J12 * J01 - J11 * J02;
+          const double detJ = J00 * term0 - J10 * term1 + J20 * term2;
+          const double term = 1.0 * detJ * tab.gw[ig] * tab.gw[jg] * tab.gw[kg];
+#pragma unroll
+          for (int i = 0; i < 8; ++i) F[i] += tab.psi[8 * q + i] * term;
+        }
   }
 }
 
