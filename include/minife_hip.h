@@ -226,7 +226,8 @@ int mfe_cg_solve_mixed_ex(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_
  * cg_solve (sums start at +0.0, plus workgroup partials of x.y); variant 2 =
  * the FP32 kernel inside mfe_cg_solve_mixed's inner CG (x is used as its
  * FP32 copy, converted before the timed launches).  Variants 3 and 4 are 1
- * and 2 on the CSR kernels whatever A's plan (for comparison). */
+ * and 2 on the CSR kernels whatever A's plan (for comparison); variant 5 is
+ * 1 reading the CG's (clear) done flag first, as it does inside the solve. */
 int mfe_time_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y, int variant,
                     int reps, double* avg_ms);
 

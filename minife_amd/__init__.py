@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import sys
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -92,12 +93,11 @@ def _share_torch_hip_runtime() -> None:
             return
 
 
-def _release(obj) -> None:
+def _release(obj, _finalizing=sys.is_finalizing) -> None:
     """Destructor body: free the handle unless the interpreter is shutting
     down (the HIP runtime may already be torn down; the process exit returns
     the device memory anyway)."""
-    import sys
-    if sys.is_finalizing():
+    if _finalizing():
         return
     try:
         obj.close()

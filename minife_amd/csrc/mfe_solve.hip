@@ -1321,7 +1321,7 @@ int mfe_matvec_fp32(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y
 
 int mfe_time_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y, int variant,
                     int reps, double* avg_ms) {
-  if (!ctx || !A || !avg_ms || reps <= 0 || variant < 0 || variant > 4) return MFE_BAD_ARG;
+  if (!ctx || !A || !avg_ms || reps <= 0 || variant < 0 || variant > 5) return MFE_BAD_ARG;
   int st;
   if ((st = check_vec(ctx, x, A->nrows, "matvec x")) || (st = check_vec(ctx, y, A->nrows, "matvec y")))
     return st;
@@ -1329,7 +1329,14 @@ int mfe_time_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y
   const int64_t tiles = cdiv(n, kSpmvRows);
   if ((st = ensure_partials(ctx, tiles > 0 ? tiles : 1))) return st;
   const bool f32 = variant == 2 || variant == 4;
-  const bool csr_only = variant >= 3;
+  const bool csr_only = variant == 3 || variant == 4;
+  // variant 5: variant 1 with the CG's done flag (clear), as inside the solve
+  const int* done = nullptr;
+  if (variant == 5) {
+    int* d = reinterpret_cast<int*>(ctx->d_scalars + 8);
+    MFE_CHECK(ctx, hipMemsetAsync(d, 0, sizeof(int), ctx->stream));
+    done = d;
+  }
   if ((st = ensure_plan(ctx, A, !f32, f32))) return st;
   SpmvOp<double> op64 = spmv_op<double>(A, !csr_only);
   SpmvOp<float> op32 = spmv_op<float>(A, !csr_only);
@@ -1355,7 +1362,7 @@ int mfe_time_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y
       e = launch_spmv<double, true, kPlain>(op64, x->d, y->d, nullptr, nullptr, nullptr,
                                             ctx->stream, nullptr);
     else if (!f32)
-      e = launch_spmv<double, false, kDot>(op64, x->d, y->d, nullptr, ctx->d_partials, nullptr,
+      e = launch_spmv<double, false, kDot>(op64, x->d, y->d, nullptr, ctx->d_partials, done,
                                            ctx->stream, nullptr);
     else
       e = launch_spmv<float, false, kDot>(op32, w->v32.p, w->v32.Ap, nullptr, w->v32.partials,
