@@ -63,9 +63,11 @@ def measured_traffic(N: int, kernel: str):
     return d.get("traffic_bytes_per_launch")
 
 
-def cpu_baseline(A_dev, b_dev, n, nnz, iters: int):
-    """Serial oracle CG (restatement of ref/ cg_solve + matvec_std) on the same
-    300^3 matrix, `iters` iterations; scaled to iterations/s."""
+def cpu_baseline(A_dev, b_dev, n, nnz, iters: int, omp_iters: int):
+    """The oracle's restatement of the reference CG on the same 300^3 matrix,
+    on this host: openmp/src's form on all cores (the reported baseline,
+    `omp_iters` iterations) and ref/'s serial form (`iters` iterations);
+    both scaled to iterations/s."""
     import oracle
     if not os.path.exists(oracle.LIB_PATH):
         return None
@@ -73,16 +75,27 @@ def cpu_baseline(A_dev, b_dev, n, nnz, iters: int):
     A = oracle.CSR(np.arange(n, dtype=np.int32), h["row_offsets"], h["cols"], h["vals"])
     b = b_dev.to_numpy()
     del h
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    ro = oracle.cg_solve_omp(A, b, np.zeros(n), max_iter=omp_iters, threads=threads)
+    dto = time.perf_counter() - t0
     t0 = time.perf_counter()
     r = oracle.cg_solve(A, b, np.zeros(n), max_iter=iters, want_times=True)
     dt = time.perf_counter() - t0
     t = r["times"]
     mv_s = t[2] / (r["num_iters"] + 1)
+    mv_o = ro["times"][2] / (ro["num_iters"] + 1)
     return {
-        "value": r["num_iters"] / dt, "unit": "CG iterations/s", "cores": 1, "kind": "port",
-        "sample": f"{r['num_iters']} of 200 FP64 CG iterations on the same 300^3 matrix "
-                  f"(oracle serial restatement of ref/ cg_solve, {dt:.1f} s)",
-        "spmv_GBs": spmv_bytes(n, nnz) / mv_s / 1e9,
+        "value": ro["num_iters"] / dto, "unit": "CG iterations/s", "cores": ro["threads"],
+        "kind": "port",
+        "sample": f"{ro['num_iters']} of 200 FP64 CG iterations on the same 300^3 matrix "
+                  f"(oracle restatement of openmp/src cg_solve, {ro['threads']} threads, "
+                  f"{dto:.1f} s)",
+        "spmv_GBs": spmv_bytes(n, nnz) / mv_o / 1e9,
+        "serial": {"value": r["num_iters"] / dt, "cores": 1,
+                   "sample": f"{r['num_iters']} iterations of ref/ cg_solve's serial form "
+                             f"({dt:.1f} s)",
+                   "spmv_GBs": spmv_bytes(n, nnz) / mv_s / 1e9},
     }
 
 
@@ -187,7 +200,9 @@ def main():
     ap.add_argument("--n", type=int, default=300, help="elements per side")
     ap.add_argument("--iters", type=int, default=200, help="CG iterations per step")
     ap.add_argument("--spmv-reps", type=int, default=50)
-    ap.add_argument("--cpu-iters", type=int, default=10)
+    ap.add_argument("--cpu-iters", type=int, default=10, help="serial CPU baseline iterations")
+    ap.add_argument("--cpu-omp-iters", type=int, default=50,
+                    help="OpenMP CPU baseline iterations (all host cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tol-run", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
@@ -350,7 +365,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        cpu = cpu_baseline(A, b, n, nnz, args.cpu_iters)
+        cpu = cpu_baseline(A, b, n, nnz, args.cpu_iters, args.cpu_omp_iters)
         if cpu:
             log(f"cpu baseline {cpu['value']:.3f} it/s ({cpu['sample']})")
 

@@ -66,6 +66,10 @@ def lib():
     L.orc_cg_solve.argtypes = [_i64, _i32p, _i32p, _f64p, _f64p, _f64p, ctypes.c_int,
                                ctypes.c_double, ctypes.POINTER(ctypes.c_int),
                                ctypes.POINTER(ctypes.c_double), vp, vp]
+    L.orc_cg_solve_omp.argtypes = [_i64, _i32p, _i32p, _f64p, _f64p, _f64p, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_double), vp]
+    L.orc_max_threads.restype = ctypes.c_int
     L.orc_element_gradient.argtypes = [_i32p, _f64p, _f64p, _f64p]
     L.orc_soln.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int, ctypes.c_int]
     L.orc_soln.restype = ctypes.c_double
@@ -211,6 +215,22 @@ def cg_solve(A: CSR, b: np.ndarray, x: np.ndarray | None = None, max_iter: int =
     if want_times:
         out["times"] = times
     return out
+
+
+def cg_solve_omp(A: CSR, b: np.ndarray, x: np.ndarray | None = None, max_iter: int = 200,
+                 tol: float = float(np.finfo(np.float64).eps), threads: int = 0):
+    """openmp/src's CG (OpenMP loops, reduction dots) for the multi-core CPU
+    baseline.  Returns dict(x, num_iters, normr, status, times, threads)."""
+    if x is None:
+        x = np.zeros(A.nrows, np.float64)
+    times = np.zeros(5, np.float64)
+    ni = ctypes.c_int(0)
+    nr = ctypes.c_double(0)
+    st = lib().orc_cg_solve_omp(A.nrows, A.row_offsets, A.cols, A.vals, b, x, int(max_iter),
+                                float(tol), int(threads), ctypes.byref(ni), ctypes.byref(nr),
+                                times.ctypes.data_as(ctypes.c_void_p))
+    return dict(x=x, num_iters=ni.value, normr=nr.value, status=st, times=times,
+                threads=int(threads) if threads > 0 else lib().orc_max_threads())
 
 
 def soln(x, y, z, max_p=300, max_q=300) -> float:

@@ -11,6 +11,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
@@ -424,6 +425,88 @@ void orc_waxpby(int64_t n, double alpha, const double* x, double beta, const dou
                 double* w) {
   for (int64_t i = 0; i < n; ++i) w[i] = alpha * x[i] + beta * y[i];
 }
+
+/* openmp/src restatement, for the multi-core CPU timing baseline only:
+ * the same CG with the rows/entries split over OpenMP threads (matvec
+ * `parallel for` over rows, openmp/src/SparseMatrix_functions.hpp:518-540;
+ * waxpby `parallel for`, openmp/src/Vector_functions.hpp:150-185; dot
+ * `parallel for reduction(+)`, :240-252).  The dot's sum order then depends
+ * on the thread count, as in the reference's openmp build. */
+static void omp_matvec(int64_t n, const int* ro, const int* cols, const double* vals,
+                       const double* x, double* y) {
+  const double beta = 0;
+#pragma omp parallel for schedule(static)
+  for (int64_t row = 0; row < n; ++row) {
+    double sum = beta * y[row];
+    for (int i = ro[row]; i < ro[row + 1]; ++i) sum += vals[i] * x[cols[i]];
+    y[row] = sum;
+  }
+}
+
+static double omp_dot(int64_t n, const double* x, const double* y) {
+  double r = 0;
+#pragma omp parallel for schedule(static) reduction(+ : r)
+  for (int64_t i = 0; i < n; ++i) r += x[i] * y[i];
+  return r;
+}
+
+static void omp_waxpby(int64_t n, double alpha, const double* x, double beta, const double* y,
+                       double* w) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) w[i] = alpha * x[i] + beta * y[i];
+}
+
+int orc_cg_solve_omp(int64_t n, const int* ro, const int* cols, const double* vals,
+                     const double* b, double* x, int max_iter, double tolerance, int nthreads,
+                     int* num_iters, double* normr_out, double* times) {
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  double tW = 0, tD = 0, tM = 0, t0;
+  double total = now_s();
+  double* r = calloc((size_t)n, sizeof(double));
+  double* p = calloc((size_t)n, sizeof(double));
+  double* Ap = calloc((size_t)n, sizeof(double));
+  if (!r || !p || !Ap) { free(r); free(p); free(Ap); return 2; }
+  double normr = 0, rtrans = 0, oldrtrans = 0;
+  int status = 0;
+  *num_iters = 0;
+  t0 = now_s(); omp_waxpby(n, 1.0, x, 0.0, x, p); tW += now_s() - t0;
+  t0 = now_s(); omp_matvec(n, ro, cols, vals, p, Ap); tM += now_s() - t0;
+  t0 = now_s(); omp_waxpby(n, 1.0, b, -1.0, Ap, r); tW += now_s() - t0;
+  t0 = now_s(); rtrans = omp_dot(n, r, r); tD += now_s() - t0;
+  normr = sqrt(rtrans);
+  double brkdown_tol = DBL_EPSILON;
+  for (int k = 1; k <= max_iter && normr > tolerance; ++k) {
+    if (k == 1) {
+      t0 = now_s(); omp_waxpby(n, 1.0, r, 0.0, r, p); tW += now_s() - t0;
+    } else {
+      oldrtrans = rtrans;
+      t0 = now_s(); rtrans = omp_dot(n, r, r); tD += now_s() - t0;
+      const double beta = rtrans / oldrtrans;
+      t0 = now_s(); omp_waxpby(n, 1.0, r, beta, p, p); tW += now_s() - t0;
+    }
+    normr = sqrt(rtrans);
+    t0 = now_s(); omp_matvec(n, ro, cols, vals, p, Ap); tM += now_s() - t0;
+    t0 = now_s(); const double p_ap_dot = omp_dot(n, Ap, p); tD += now_s() - t0;
+    if (p_ap_dot < brkdown_tol) {
+      if (p_ap_dot < 0) { status = 1; break; }
+      brkdown_tol = 0.1 * p_ap_dot;
+    }
+    const double alpha = rtrans / p_ap_dot;
+    t0 = now_s();
+    omp_waxpby(n, 1.0, x, alpha, p, x);
+    omp_waxpby(n, 1.0, r, -alpha, Ap, r);
+    tW += now_s() - t0;
+    *num_iters = k;
+  }
+  if (times) {
+    times[0] = tW; times[1] = tD; times[2] = tM; times[3] = 0; times[4] = now_s() - total;
+  }
+  *normr_out = normr;
+  free(r); free(p); free(Ap);
+  return status;
+}
+
+int orc_max_threads(void) { return omp_get_max_threads(); }
 
 /* cg_solve.hpp:49-67 */
 static int breakdown(int64_t n, double inner, const double* v, const double* w) {

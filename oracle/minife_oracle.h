@@ -105,6 +105,15 @@ int orc_cg_solve(int64_t nrows, const int* row_offsets, const int* cols,
                  double tolerance, int* num_iters, double* normr, double* history,
                  double* times);
 
+/* The same CG with OpenMP loops as openmp/src (matvec, waxpby, reduction
+ * dot): the multi-core CPU timing baseline.  nthreads <= 0: OpenMP default.
+ * Pinned by the serial solve (same iterate up to the dot's summation order). */
+int orc_cg_solve_omp(int64_t nrows, const int* row_offsets, const int* cols,
+                     const double* vals, const double* b, double* x, int max_iter,
+                     double tolerance, int nthreads, int* num_iters, double* normr,
+                     double* times);
+int orc_max_threads(void);
+
 /* Adjoint element gradient (no reference counterpart): grad[e] =
  * -lambda_e^T K_e(1) utilde_e with K_e(1) = diffusionMatrix_symm at k = 1;
  * per element: for i: t = sum_j K_ij utilde_j; s += lambda_i t; grad = -s.

@@ -398,3 +398,16 @@ def test_adjoint_uniform_scaling_identity(oracle_mod):
     lhs = float(np.dot(kappa, g))
     rhs_val = -float(np.dot(lam, P["b_prebc"]))
     assert abs(lhs - rhs_val) <= 1e-9 * abs(rhs_val), (lhs, rhs_val)
+
+
+def test_openmp_cg_matches_serial(oracle_mod):
+    """The OpenMP CG (CPU baseline) runs the serial CG's iteration: the same
+    iteration count and a solution within the dot's reordering error."""
+    P = oracle_mod.build_problem(14, 11, 9)
+    r1 = oracle_mod.cg_solve(P["A"], P["b"], max_iter=500)
+    r4 = oracle_mod.cg_solve_omp(P["A"], P["b"], max_iter=500, threads=4)
+    assert r4["threads"] == 4 and r4["status"] == 0
+    assert abs(r4["num_iters"] - r1["num_iters"]) <= 2
+    assert np.max(np.abs(r4["x"] - r1["x"])) <= 1e-10 * np.max(np.abs(r1["x"]))
+    r1t = oracle_mod.cg_solve_omp(P["A"], P["b"], max_iter=5, threads=1)
+    assert np.array_equal(r1t["x"], oracle_mod.cg_solve(P["A"], P["b"], max_iter=5)["x"])
