@@ -115,6 +115,11 @@ int mfe_perform_element_loop(mfe_ctx* ctx, const int global_box[6], const int el
 /* assemble_FE_data(mesh, A, b, params), assemble_FE_data.hpp:38-73. */
 int mfe_assemble_FE_data(mfe_ctx* ctx, const int global_box[6], const int local_box[6],
                          mfe_csr* A, mfe_vec* b, const mfe_vec* kappa);
+/* assemble_FE_data with the element source Q (sourceVector's Q, Hex8.hpp:379;
+ * the reference uses 1.0, which mfe_assemble_FE_data passes; 0 gives the
+ * source-free problem).  Q = 1.0 gives the same bits as mfe_assemble_FE_data. */
+int mfe_assemble_FE_data_ex(mfe_ctx* ctx, const int global_box[6], const int local_box[6],
+                            mfe_csr* A, mfe_vec* b, const mfe_vec* kappa, double source);
 
 /* impose_dirichlet(value, A, b, gnx, gny, gnz, bc_rows),
  * SparseMatrix_functions.hpp:431-480: bc_rows is the reference's std::set as a
@@ -294,6 +299,11 @@ double mfe_soln(double x, double y, double z, int max_p, int max_q);
  * *max_err (nullable) receives that error. */
 int mfe_verify_solution(mfe_ctx* ctx, const int global_box[6], const mfe_vec* x,
                         double tolerance, int verify_whole_domain, double* max_err);
+/* verify_solution for either choice of the u = 1 face: hot_face 1 = x=1 (the
+ * reference), 0 = x=0, checked against the mirrored series soln(1-x, y, z). */
+int mfe_verify_solution_ex(mfe_ctx* ctx, const int global_box[6], const mfe_vec* x,
+                           double tolerance, int verify_whole_domain, int hot_face,
+                           double* max_err);
 
 #ifdef __cplusplus
 }

@@ -46,6 +46,7 @@ EXPORTS = [
     "mfe_vec_create", "mfe_vec_destroy", "mfe_vec_size", "mfe_vec_upload", "mfe_vec_download",
     "mfe_vec_fill", "mfe_vec_wrap", "mfe_element_gradient",
     "mfe_generate_matrix_structure", "mfe_perform_element_loop", "mfe_assemble_FE_data",
+    "mfe_assemble_FE_data_ex", "mfe_verify_solution_ex",
     "mfe_impose_dirichlet", "mfe_make_local_matrix", "mfe_spmv_plan",
     "mfe_matvec", "mfe_matvec_and_dot", "mfe_dot", "mfe_dot_replay", "mfe_waxpby",
     "mfe_cg_solve", "mfe_cg_solve_mixed", "mfe_cg_solve_mixed_ex", "mfe_matvec_fp32", "mfe_time_matvec", "mfe_soln",
@@ -139,6 +140,8 @@ def lib() -> ctypes.CDLL:
         "mfe_generate_matrix_structure": ([_vp, _int6, _int6, _vp], ctypes.c_int),
         "mfe_perform_element_loop": ([_vp, _int6, _int6, _vp, _vp, _vp], ctypes.c_int),
         "mfe_assemble_FE_data": ([_vp, _int6, _int6, _vp, _vp, _vp], ctypes.c_int),
+        "mfe_assemble_FE_data_ex": ([_vp, _int6, _int6, _vp, _vp, _vp, ctypes.c_double],
+                                    ctypes.c_int),
         "mfe_impose_dirichlet": ([_vp, ctypes.c_double, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, _vp, _i64], ctypes.c_int),
         "mfe_make_local_matrix": ([_vp, _vp], ctypes.c_int),
@@ -180,6 +183,8 @@ def lib() -> ctypes.CDLL:
         "mfe_batch_timing": ([_vp, _vp, _vp], ctypes.c_int),
         "mfe_verify_solution": ([_vp, _int6, _vp, ctypes.c_double, ctypes.c_int,
                                  ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "mfe_verify_solution_ex": ([_vp, _int6, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -367,6 +372,10 @@ class Parameters:
     nz: int = -1
     verify_solution: int = 0
     name: str = ""
+    # not in the reference (its problem is fixed): the element source Q and
+    # the face held at u = 1 (1: x = 1 as the reference, 0: x = 0)
+    source: float = 1.0
+    hot_face: int = 1
     extra: dict = field(default_factory=dict)
 
     def __post_init__(self):
@@ -381,10 +390,12 @@ class simple_mesh_description:
 
     bc_rows_0: nodes on the faces x=0, y=0, z=0, y=1, z=1 (prescribed 0);
     bc_rows_1: nodes on the face x=1 (prescribed 1).  Sorted int32 arrays
-    (the reference's std::set); one rank so node id == row.
+    (the reference's std::set); one rank so node id == row.  hot_face=0 swaps
+    the roles of x=0 and x=1 (u = 1 on x = 0).
     """
 
-    def __init__(self, global_box, local_box=None):
+    def __init__(self, global_box, local_box=None, hot_face: int = 1):
+        self.hot_face = int(hot_face)
         self.global_box = np.asarray(global_box, dtype=np.int64).reshape(6)
         self.local_box = self.global_box.copy() if local_box is None else \
             np.asarray(local_box, dtype=np.int64).reshape(6)
@@ -397,8 +408,9 @@ class simple_mesh_description:
         iz, iy, ix = np.meshgrid(np.arange(gz), np.arange(gy), np.arange(gx), indexing="ij")
         ids = (ix + gx * iy + gx * gy * iz).reshape(-1)
         ix, iy, iz = ix.reshape(-1), iy.reshape(-1), iz.reshape(-1)
-        in0 = (ix == 0) | (iy == 0) | (iz == 0) | (iy == gy - 1) | (iz == gz - 1)
-        in1 = ix == gx - 1
+        hot, cold = (gx - 1, 0) if self.hot_face else (0, gx - 1)
+        in0 = (ix == cold) | (iy == 0) | (iz == 0) | (iy == gy - 1) | (iz == gz - 1)
+        in1 = ix == hot
         self.bc_rows_0 = ids[in0].astype(np.int32)
         self.bc_rows_1 = ids[in1].astype(np.int32)
 
@@ -424,9 +436,14 @@ def perform_element_loop(mesh, elem_box, A: CSRMatrix, b: Vector, params=None, k
     A.ctx.check(st, "perform_element_loop")
 
 
-def assemble_FE_data(mesh, A: CSRMatrix, b: Vector, params=None, kappa=None):
-    st = lib().mfe_assemble_FE_data(A.ctx.h, _box6(mesh.global_box), _box6(mesh.local_box), A.h,
-                                    b.h, kappa.h if kappa is not None else None)
+def assemble_FE_data(mesh, A: CSRMatrix, b: Vector, params=None, kappa=None, source=None):
+    """assemble_FE_data.hpp:38-73; source (or params.source) is the element
+    source Q, 1.0 as the reference."""
+    if source is None:
+        source = params.source if params is not None else 1.0
+    st = lib().mfe_assemble_FE_data_ex(A.ctx.h, _box6(mesh.global_box), _box6(mesh.local_box),
+                                       A.h, b.h, kappa.h if kappa is not None else None,
+                                       float(source))
     A.ctx.check(st, "assemble_FE_data")
 
 
@@ -676,25 +693,26 @@ def verify_solution(mesh: simple_mesh_description, x: Vector, tolerance: float =
                     verify_whole_domain: bool = False):
     """Returns (rc, max_abs_err); rc = 0 when within tolerance."""
     e = ctypes.c_double(0)
-    rc = lib().mfe_verify_solution(x.ctx.h, _box6(mesh.global_box), x.h, float(tolerance),
-                                   int(bool(verify_whole_domain)), ctypes.byref(e))
+    rc = lib().mfe_verify_solution_ex(x.ctx.h, _box6(mesh.global_box), x.h, float(tolerance),
+                                      int(bool(verify_whole_domain)),
+                                      int(getattr(mesh, "hot_face", 1)), ctypes.byref(e))
     if rc not in (0, 1):
         x.ctx.check(rc, "verify_solution")
     return rc, e.value
 
 
 def build_problem(nx: int, ny: int | None = None, nz: int | None = None, ctx: Context | None = None,
-                  kappa: Vector | None = None):
+                  kappa: Vector | None = None, source: float = 1.0, hot_face: int = 1):
     """Structure -> assembly -> Dirichlet (u=0 faces, then u=1 face) -> make_local_matrix,
     the order of driver.hpp:159-230.  Returns (mesh, A, b)."""
     ny = nx if ny is None else ny
     nz = ny if nz is None else nz
-    mesh = simple_mesh_description([0, nx, 0, ny, 0, nz])
+    mesh = simple_mesh_description([0, nx, 0, ny, 0, nz], hot_face=hot_face)
     A = CSRMatrix(ctx)
     if generate_matrix_structure(mesh, A) != 0:
         raise MfeError(MFE_OOM, "generate_matrix_structure ran out of device memory")
     b = Vector(A.nrows, A.ctx)
-    assemble_FE_data(mesh, A, b, kappa=kappa)
+    assemble_FE_data(mesh, A, b, kappa=kappa, source=source)
     impose_dirichlet(0.0, A, b, nx + 1, ny + 1, nz + 1, mesh.bc_rows_0)
     impose_dirichlet(1.0, A, b, nx + 1, ny + 1, nz + 1, mesh.bc_rows_1)
     make_local_matrix(A)

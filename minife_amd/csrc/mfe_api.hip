@@ -373,6 +373,11 @@ double mfe_soln(double x, double y, double z, int max_p, int max_q) {
 // verify_solution.hpp:55-174.  Only the checked nodes are copied to the host.
 int mfe_verify_solution(mfe_ctx* ctx, const int g[6], const mfe_vec* xv, double tolerance,
                         int whole, double* max_err) {
+  return mfe_verify_solution_ex(ctx, g, xv, tolerance, whole, 1, max_err);
+}
+
+int mfe_verify_solution_ex(mfe_ctx* ctx, const int g[6], const mfe_vec* xv, double tolerance,
+                           int whole, int hot_face, double* max_err) {
   if (!ctx || !g || !xv) return MFE_BAD_ARG;
   const int gx = g[1] + 1, gy = g[3] + 1, gz = g[5] + 1;
   if (g[0] != 0 || g[2] != 0 || g[4] != 0 || (int64_t)gx * gy * gz != xv->n)
@@ -402,11 +407,12 @@ int mfe_verify_solution(mfe_ctx* ctx, const int g[6], const mfe_vec* xv, double 
         } else {
           MFE_CHECK(ctx, hipMemcpy(&computed, xv->d + id, sizeof(double), hipMemcpyDeviceToHost));
         }
+        // hot_face 0 (u = 1 on x = 0): the mirrored solution soln(1 - x, y, z)
         double a;
-        if (x == 1.0) a = 1;
-        else if (x == 0.0 || y == 0.0 || z == 0.0) a = 0;
+        if (hot_face ? x == 1.0 : x == 0.0) a = 1;
+        else if (x == 0.0 || x == 1.0 || y == 0.0 || z == 0.0) a = 0;
         else if (y == 1.0 || z == 1.0) a = 0;
-        else a = mfe_soln(x, y, z, 300, 300);
+        else a = mfe_soln(hot_face ? x : 1.0 - x, y, z, 300, 300);
         double err = std::abs(a - computed);
         if (err > maxe) maxe = err;
       }

@@ -26,7 +26,7 @@ int csr_alloc(mfe_ctx* ctx, mfe_csr* A, int64_t nrows, int64_t nnz);  // mfe_api
 __global__ __launch_bounds__(kBlock) void element_kernel(int Nx, int Ny, int Nz, double hx,
                                                          double hy, double hz, HexTables tab,
                                                          const double* __restrict__ kappa,
-                                                         double* __restrict__ Kout,
+                                                         double Q, double* __restrict__ Kout,
                                                          double* __restrict__ Fout, int64_t ne) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
@@ -40,7 +40,7 @@ __global__ __launch_bounds__(kBlock) void element_kernel(int Nx, int Ny, int Nz,
   const double c[24] = {x0, y0, z0, x1, y0, z0, x1, y1, z0, x0, y1, z0,
                         x0, y0, z1, x1, y0, z1, x1, y1, z1, x0, y1, z1};
   double K[36], F[8];
-  hex8_element(c, tab, kappa ? kappa[e] : 1.0, K, F);
+  hex8_element(c, tab, kappa ? kappa[e] : 1.0, K, F, Q);
 #pragma unroll
   for (int i = 0; i < 36; ++i) Kout[(int64_t)i * ne + e] = K[i];
 #pragma unroll
@@ -265,8 +265,16 @@ int mfe_generate_matrix_structure(mfe_ctx* ctx, const int gbox[6], const int lbo
   return plan_spmv(ctx, A);
 }
 
+static int element_loop(mfe_ctx* ctx, const int gbox[6], const int ebox[6], mfe_csr* A,
+                        mfe_vec* b, const mfe_vec* kappa, double source);
+
 int mfe_perform_element_loop(mfe_ctx* ctx, const int gbox[6], const int ebox[6], mfe_csr* A,
                              mfe_vec* b, const mfe_vec* kappa) {
+  return element_loop(ctx, gbox, ebox, A, b, kappa, 1.0);
+}
+
+static int element_loop(mfe_ctx* ctx, const int gbox[6], const int ebox[6], mfe_csr* A,
+                        mfe_vec* b, const mfe_vec* kappa, double source) {
   if (!ctx || !A || !b || !box_ok(gbox) || !ebox) return MFE_BAD_ARG;
   if (!same_box(gbox, ebox))
     return fail(ctx, MFE_UNSUPPORTED, "one rank: the element box must be the global box");
@@ -290,7 +298,8 @@ int mfe_perform_element_loop(mfe_ctx* ctx, const int gbox[6], const int ebox[6],
   }
   int st = MFE_OK;
   element_kernel<<<(unsigned)cdiv(ne, kBlock), kBlock, 0, ctx->stream>>>(
-      Nx, Ny, Nz, 1.0 / Nx, 1.0 / Ny, 1.0 / Nz, tab, kappa ? kappa->d : nullptr, Kb, Fb, ne);
+      Nx, Ny, Nz, 1.0 / Nx, 1.0 / Ny, 1.0 / Nz, tab, kappa ? kappa->d : nullptr, source, Kb, Fb,
+      ne);
   hipError_t le = hipGetLastError();
   if (le == hipSuccess) {
     gather_kernel<<<(unsigned)cdiv(n, kBlock), kBlock, 0, ctx->stream>>>(Nx, Ny, Nz,
@@ -314,6 +323,14 @@ int mfe_assemble_FE_data(mfe_ctx* ctx, const int gbox[6], const int lbox[6], mfe
     return fail(ctx, MFE_UNSUPPORTED, "one rank: local box must equal the global box");
   // One rank: the ghost layer of assemble_FE_data.hpp:59-72 is empty.
   return mfe_perform_element_loop(ctx, gbox, lbox, A, b, kappa);
+}
+
+int mfe_assemble_FE_data_ex(mfe_ctx* ctx, const int gbox[6], const int lbox[6], mfe_csr* A,
+                            mfe_vec* b, const mfe_vec* kappa, double source) {
+  if (!ctx || !box_ok(gbox) || !lbox) return MFE_BAD_ARG;
+  if (!same_box(gbox, lbox))
+    return fail(ctx, MFE_UNSUPPORTED, "one rank: local box must equal the global box");
+  return element_loop(ctx, gbox, lbox, A, b, kappa, source);
 }
 
 int mfe_impose_dirichlet(mfe_ctx* ctx, double value, mfe_csr* A, mfe_vec* b, int gnx, int gny,

@@ -76,7 +76,7 @@ __host__ __device__ constexpr int packed(int a, int b) {
 // holds the register footprint near the 44 accumulators; same arithmetic.
 template <bool LEAN = false>
 __device__ __forceinline__ void hex8_element(const double c[24], const HexTables& tab, double k,
-                                             double K[36], double F[8]) {
+                                             double K[36], double F[8], double Q = 1.0) {
 #pragma unroll
   for (int i = 0; i < 36; ++i) K[i] = 0;
 #pragma unroll
@@ -135,8 +135,8 @@ __device__ __forceinline__ void hex8_element(const double c[24], const HexTables
             K[off++] += kdw * ((dx[m] * dx[n]) + (dy[m] * dy[n]) + (dz[m] * dz[n]));
         }
         if (!LEAN) {
-          // sourceVector: term = Q*detJ*wi*wj*wk with Q = 1.0 (Hex8.hpp:379,402).
-          const double term = 1.0 * detJ * wi * wj * wk;
+          // sourceVector: term = Q*detJ*wi*wj*wk, reference Q = 1.0 (Hex8.hpp:379,402).
+          const double term = Q * detJ * wi * wj * wk;
 #pragma unroll
           for (int i = 0; i < 8; ++i) F[i] += tab.psi[8 * q + i] * term;
         }
@@ -165,7 +165,7 @@ __device__ __forceinline__ void hex8_element(const double c[24], const HexTables
           const double term1 = J22 * J01 - J21 * J02;
           const double term2 = J12 * J01 - J11 * J02;
           const double detJ = J00 * term0 - J10 * term1 + J20 * term2;
-          const double term = 1.0 * detJ * tab.gw[ig] * tab.gw[jg] * tab.gw[kg];
+          const double term = Q * detJ * tab.gw[ig] * tab.gw[jg] * tab.gw[kg];
 #pragma unroll
           for (int i = 0; i < 8; ++i) F[i] += tab.psi[8 * q + i] * term;
         }

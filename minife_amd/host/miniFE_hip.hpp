@@ -82,6 +82,10 @@ struct Parameters {
   std::string name;
   int elem_group_size = 1, use_elem_mat_fields = 1, verify_solution = 0;
   int device = 0, num_devices = 2, skip_device = 9999, numa = 1;
+  // not in the reference (its problem is fixed): element source Q (1.0 as
+  // Hex8.hpp:379) and the face held at u = 1 (1: x = 1 as the reference, 0: x = 0)
+  double source = 1.0;
+  int hot_face = 1;
 };
 
 template <typename GlobalOrdinal>
@@ -90,7 +94,8 @@ class simple_mesh_description {
   // One rank: local_box must equal global_box.  bc_rows_0 holds the nodes of
   // the faces x=0, y=0, z=0, y=1, z=1 and bc_rows_1 those of x=1
   // (simple_mesh_description.hpp:77-224); node id == row.
-  simple_mesh_description(const Box& global_box_in, const Box& local_box_in) {
+  simple_mesh_description(const Box& global_box_in, const Box& local_box_in, int hot_face_in = 1)
+      : hot_face(hot_face_in) {
     static_assert(std::is_same<GlobalOrdinal, int>::value, "GlobalOrdinal must be int");
     for (int i = 0; i < 6; ++i) {
       global_box.ranges[i] = global_box_in.ranges[i];
@@ -99,17 +104,19 @@ class simple_mesh_description {
         throw std::runtime_error("miniFE (HIP): one rank per GPU; local_box must equal global_box");
     }
     const int gx = global_box[0][1] + 1, gy = global_box[1][1] + 1, gz = global_box[2][1] + 1;
+    const int hot = hot_face ? gx - 1 : 0, cold = hot_face ? 0 : gx - 1;
     for (int iz = 0; iz < gz; ++iz)
       for (int iy = 0; iy < gy; ++iy)
         for (int ix = 0; ix < gx; ++ix) {
           const GlobalOrdinal id = ix + gx * iy + gx * gy * iz;
-          if (ix == 0 || iy == 0 || iz == 0 || iy == gy - 1 || iz == gz - 1) bc_rows_0.insert(id);
-          if (ix == gx - 1) bc_rows_1.insert(id);
+          if (ix == cold || iy == 0 || iz == 0 || iy == gy - 1 || iz == gz - 1) bc_rows_0.insert(id);
+          if (ix == hot) bc_rows_1.insert(id);
         }
   }
   GlobalOrdinal map_id_to_row(const GlobalOrdinal& id) const { return id; }
   GlobalOrdinal max_row_in_map() const { return 0; }
 
+  int hot_face = 1;  // 1: u = 1 on x = 1 (the reference); 0: on x = 0
   std::set<GlobalOrdinal> bc_rows_0;
   std::set<GlobalOrdinal> bc_rows_1;
   Box global_box;
@@ -211,9 +218,9 @@ void perform_element_loop(
 
 template <typename MatrixType, typename VectorType>
 void assemble_FE_data(const simple_mesh_description<typename MatrixType::GlobalOrdinalType>& mesh,
-                      MatrixType& A, VectorType& b, Parameters& /*params*/) {
-  check(mfe_assemble_FE_data(context(), mesh.global_box.ranges, mesh.local_box.ranges, A.h, b.h,
-                             nullptr),
+                      MatrixType& A, VectorType& b, Parameters& params) {
+  check(mfe_assemble_FE_data_ex(context(), mesh.global_box.ranges, mesh.local_box.ranges, A.h,
+                                b.h, nullptr, params.source),
         "assemble_FE_data");
 }
 
@@ -306,8 +313,8 @@ int verify_solution(const simple_mesh_description<typename VectorType::GlobalOrd
                     const VectorType& x, double tolerance, bool verify_whole_domain = false,
                     double* max_err = nullptr) {
   double e = 0;
-  const int rc = mfe_verify_solution(context(), mesh.global_box.ranges, x.h, tolerance,
-                                     verify_whole_domain ? 1 : 0, &e);
+  const int rc = mfe_verify_solution_ex(context(), mesh.global_box.ranges, x.h, tolerance,
+                                        verify_whole_domain ? 1 : 0, mesh.hot_face, &e);
   if (rc != 0 && rc != 1) check(rc, "verify_solution");
   if (max_err) *max_err = e;
   return rc;

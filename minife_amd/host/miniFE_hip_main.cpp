@@ -8,7 +8,9 @@
 // Extra keys (no reference counterpart): max_iters (default 200), tolerance
 // (default DBL_EPSILON), relative_tol=1, replay=1 (reference-exact CG
 // trajectory), eager=1 (per-kernel CG timers), mixed=1 (FP32-inner / FP64
-// refinement solve to outer_tol, default 1e-10).
+// refinement solve to outer_tol, default 1e-10; mixed=2 keeps the FP32
+// search direction across refinements), bc_face=x0 (u = 1 on x = 0 instead
+// of x = 1), source=0 (no element source; the reference's Q is 1).
 #include <sys/time.h>
 
 #include <cstdio>
@@ -111,7 +113,9 @@ int main(int argc, char** argv) {
   opt.replay = param(argc, argv, "replay", 0) != 0;
   opt.eager = param(argc, argv, "eager", 0) != 0;
   opt.relative_tol = param(argc, argv, "relative_tol", 0) != 0;
-  const bool mixed = param(argc, argv, "mixed", 0) != 0;
+  const int mixed = param(argc, argv, "mixed", 0);
+  params.source = param(argc, argv, "source", 1.0);
+  params.hot_face = arg_value(argc, argv, "bc_face") == "x0" ? 0 : 1;
   const double outer_tol = param(argc, argv, "outer_tol", 1e-10);
 
   const double start_time = mytimer();
@@ -125,6 +129,8 @@ int main(int argc, char** argv) {
   doc.get("Global Run Parameters")->get("dimensions")->add("ny", params.ny);
   doc.get("Global Run Parameters")->get("dimensions")->add("nz", params.nz);
   doc.get("Global Run Parameters")->add("load_imbalance", params.load_imbalance);
+  if (params.source != 1.0) doc.get("Global Run Parameters")->add("source", params.source);
+  if (params.hot_face != 1) doc.get("Global Run Parameters")->add("u=1 face", "x=0");
   doc.get("Global Run Parameters")->add("mv_overlap_comm_comp", "0 (no)");
   doc.get("Global Run Parameters")->add("number of processors", 1);
   char devname[256] = {0};
@@ -150,7 +156,7 @@ int main(int argc, char** argv) {
   std::cout << "creating/filling mesh...";
   std::cout.flush();
   double t_start = mytimer(), t0 = mytimer();
-  simple_mesh_description<int> mesh(global_box, my_box);
+  simple_mesh_description<int> mesh(global_box, my_box, params.hot_face);
   double mesh_fill = mytimer() - t0;
   double t_total = mytimer() - t_start;
   std::cout << mesh_fill << "s, total time: " << t_total << std::endl;
@@ -224,8 +230,10 @@ int main(int argc, char** argv) {
     std::cout << "Starting mixed-precision CG (FP32 inner, FP64 refinement) ... " << std::endl;
     int outer = 0, inner = 0;
     double rel = 0, t[5] = {0, 0, 0, 0, 0};
-    const int st = mfe_cg_solve_mixed(context(), A.h, b.h, x.h, 50, 20000, 1e-5, outer_tol,
-                                      &outer, &inner, &rel, t);
+    const bool keep = mixed == 2;
+    const int st = mfe_cg_solve_mixed_ex(context(), A.h, b.h, x.h, 200, 20000, keep ? 0.1 : 1e-5,
+                                         outer_tol, keep ? MFE_MIXED_KEEP_DIRECTION : 0u, &outer,
+                                         &inner, &rel, t);
     if (st != MFE_OK) std::cerr << "mixed solve: " << mfe_last_error(context()) << std::endl;
     num_iters = inner;
     rnorm = rel;

@@ -55,6 +55,12 @@ def lib():
     L.orc_get_elem_nodes_and_coords.argtypes = [_i32p, ctypes.c_int, _i32p, _f64p]
     L.orc_assemble_FE_data.argtypes = [_i32p, _i64, _i32p, _i32p, _f64p, _f64p, vp]
     L.orc_bc_rows.argtypes = [_i32p, ctypes.c_int, vp]
+    L.orc_bc_rows_face.argtypes = [_i32p, ctypes.c_int, ctypes.c_int, vp]
+    L.orc_bc_rows_face.restype = _i64
+    L.orc_assemble_FE_data_q.argtypes = [_i32p, _i64, _i32p, _i32p, _f64p, _f64p, vp,
+                                         ctypes.c_double]
+    L.orc_verify_solution_face.argtypes = [_i32p, _f64p, ctypes.c_double, ctypes.c_int,
+                                           ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.orc_bc_rows.restype = _i64
     L.orc_impose_dirichlet.argtypes = [ctypes.c_double, _i64, _i32p, _i32p, _f64p, _f64p,
                                        _i32p, _i64]
@@ -143,8 +149,9 @@ def elem_nodes_and_coords(nx, ny, nz, e: int):
     return ids, c
 
 
-def assemble_FE_data(nx, ny, nz, A: CSR, kappa=None, b=None) -> np.ndarray:
-    """Adds the element contributions onto A.vals and b (a new zero b if None)."""
+def assemble_FE_data(nx, ny, nz, A: CSR, kappa=None, b=None, source: float = 1.0) -> np.ndarray:
+    """Adds the element contributions onto A.vals and b (a new zero b if None);
+    source is the element source Q (reference 1.0)."""
     if b is None:
         b = np.zeros(A.nrows, np.float64)
     kp = None
@@ -152,18 +159,20 @@ def assemble_FE_data(nx, ny, nz, A: CSR, kappa=None, b=None) -> np.ndarray:
         kappa = np.ascontiguousarray(kappa, dtype=np.float64)
         assert kappa.shape == (nx * ny * nz,)
         kp = kappa.ctypes.data_as(ctypes.c_void_p)
-    rc = lib().orc_assemble_FE_data(box(nx, ny, nz), A.nrows, A.row_offsets, A.cols, A.vals,
-                                    b, kp)
+    rc = lib().orc_assemble_FE_data_q(box(nx, ny, nz), A.nrows, A.row_offsets, A.cols, A.vals,
+                                      b, kp, float(source))
     if rc:
         raise ValueError("assemble failed")
     return b
 
 
-def bc_rows(nx, ny, nz, which: int) -> np.ndarray:
+def bc_rows(nx, ny, nz, which: int, hot_face: int = 1) -> np.ndarray:
+    """bc_rows_0 (which=0) or bc_rows_1 (which=1); hot_face 1 puts u=1 on x=1
+    (reference), 0 on x=0."""
     L = lib()
-    cnt = L.orc_bc_rows(box(nx, ny, nz), which, None)
+    cnt = L.orc_bc_rows_face(box(nx, ny, nz), which, int(hot_face), None)
     out = np.empty(cnt, np.int32)
-    L.orc_bc_rows(box(nx, ny, nz), which, out.ctypes.data_as(ctypes.c_void_p))
+    L.orc_bc_rows_face(box(nx, ny, nz), which, int(hot_face), out.ctypes.data_as(ctypes.c_void_p))
     return out
 
 
@@ -237,10 +246,10 @@ def soln(x, y, z, max_p=300, max_q=300) -> float:
     return lib().orc_soln(float(x), float(y), float(z), int(max_p), int(max_q))
 
 
-def verify_solution(nx, ny, nz, x, tol=0.06, whole=False):
+def verify_solution(nx, ny, nz, x, tol=0.06, whole=False, hot_face=1):
     e = ctypes.c_double(0)
-    rc = lib().orc_verify_solution(box(nx, ny, nz), np.ascontiguousarray(x), float(tol),
-                                   int(whole), ctypes.byref(e))
+    rc = lib().orc_verify_solution_face(box(nx, ny, nz), np.ascontiguousarray(x), float(tol),
+                                        int(whole), int(hot_face), ctypes.byref(e))
     return rc, e.value
 
 
@@ -276,11 +285,11 @@ def loss_and_gradient(nx, ny, nz, kappa, u_target, rel_tol=1e-13):
     return L, element_gradient(nx, ny, nz, lam, ut), u
 
 
-def build_problem(nx, ny, nz, kappa=None):
+def build_problem(nx, ny, nz, kappa=None, source=1.0, hot_face=1):
     """Structure -> assembly -> Dirichlet (0 then 1), as driver.hpp:181-218."""
     A = generate_matrix_structure(nx, ny, nz)
-    b = assemble_FE_data(nx, ny, nz, A, kappa)
+    b = assemble_FE_data(nx, ny, nz, A, kappa, source=source)
     A_pre, b_pre = A.copy(), b.copy()
-    impose_dirichlet(0.0, A, b, bc_rows(nx, ny, nz, 0))
-    impose_dirichlet(1.0, A, b, bc_rows(nx, ny, nz, 1))
+    impose_dirichlet(0.0, A, b, bc_rows(nx, ny, nz, 0, hot_face))
+    impose_dirichlet(1.0, A, b, bc_rows(nx, ny, nz, 1, hot_face))
     return dict(A=A, b=b, A_prebc=A_pre, b_prebc=b_pre)

@@ -295,9 +295,21 @@ static void sum_into_row(int row_len, const int* row_idx, double* row_coefs, int
   }
 }
 
+static int element_loop_q(const int gbox[6], const int eb[6], int64_t nrows,
+                          const int* row_offsets, const int* cols, double* vals, double* b,
+                          const double* kappa, double Q);
+
 int orc_perform_element_loop(const int gbox[6], const int eb[6], int64_t nrows,
                              const int* row_offsets, const int* cols, double* vals,
                              double* b, const double* kappa) {
+  return element_loop_q(gbox, eb, nrows, row_offsets, cols, vals, b, kappa, 1.0);
+}
+
+/* The element loop with source Q (ref: Q = 1.0, Hex8.hpp:379; Q = 0 is the
+ * source-free variant of the miniFE problem). */
+static int element_loop_q(const int gbox[6], const int eb[6], int64_t nrows,
+                          const int* row_offsets, const int* cols, double* vals, double* b,
+                          const double* kappa, double Q) {
   double gv[192], coords[24], K[36], f[8];
   int ids[8];
   orc_compute_gradient_values(gv);
@@ -309,7 +321,7 @@ int orc_perform_element_loop(const int gbox[6], const int eb[6], int64_t nrows,
         int e = get_id(gex, gey, gez, x, y, z);
         orc_get_elem_nodes_and_coords(gbox, e, ids, coords);
         orc_diffusion_matrix_symm(coords, gv, kappa ? kappa[e] : 1.0, K);
-        orc_source_vector(coords, gv, 1.0, f);
+        orc_source_vector(coords, gv, Q, f);
         /* sum_in_symm_elem_matrix, SparseMatrix_functions.hpp:183-227 */
         int row_offset = 0;
         for (int i = 0; i < 8; ++i) {
@@ -343,14 +355,26 @@ int orc_assemble_FE_data(const int gbox[6], int64_t nrows, const int* row_offset
   return orc_perform_element_loop(gbox, gbox, nrows, row_offsets, cols, vals, b, kappa);
 }
 
+int orc_assemble_FE_data_q(const int gbox[6], int64_t nrows, const int* row_offsets,
+                           const int* cols, double* vals, double* b, const double* kappa,
+                           double Q) {
+  if (!box_is_origin(gbox)) return 1;
+  return element_loop_q(gbox, gbox, nrows, row_offsets, cols, vals, b, kappa, Q);
+}
+
 int64_t orc_bc_rows(const int gbox[6], int which, int* out) {
+  return orc_bc_rows_face(gbox, which, 1, out);
+}
+
+int64_t orc_bc_rows_face(const int gbox[6], int which, int hot_face, int* out) {
   const int gx = gbox[1] + 1, gy = gbox[3] + 1, gz = gbox[5] + 1;
+  const int hot = hot_face ? gx - 1 : 0, cold = hot_face ? 0 : gx - 1;
   int64_t cnt = 0;
   for (int iz = 0; iz < gz; ++iz)
     for (int iy = 0; iy < gy; ++iy)
       for (int ix = 0; ix < gx; ++ix) {
-        int in0 = ix == 0 || iy == 0 || iz == 0 || iy == gy - 1 || iz == gz - 1;
-        int in1 = ix == gx - 1;
+        int in0 = ix == cold || iy == 0 || iz == 0 || iy == gy - 1 || iz == gz - 1;
+        int in1 = ix == hot;
         if (which ? in1 : in0) {
           if (out) out[cnt] = get_id(gx, gy, gz, ix, iy, iz);
           ++cnt;
@@ -622,6 +646,12 @@ double orc_soln(double x, double y, double z, int max_p, int max_q) {
 
 int orc_verify_solution(const int gbox[6], const double* xv, double tolerance,
                         int whole, double* max_err) {
+  return orc_verify_solution_face(gbox, xv, tolerance, whole, 1, max_err);
+}
+
+/* hot_face 0: u = 1 on x = 0, whose Laplace solution is soln(1 - x, y, z). */
+int orc_verify_solution_face(const int gbox[6], const double* xv, double tolerance,
+                             int whole, int hot_face, double* max_err) {
   const int gx = gbox[1] + 1, gy = gbox[3] + 1, gz = gbox[5] + 1;
   double maxe = 0.0;
   int roffset = 0;
@@ -634,11 +664,12 @@ int orc_verify_solution(const int gbox[6], const double* xv, double tolerance,
         int check = whole || (fabs(x - 0.5) < 0.05 && fabs(y - 0.5) < 0.05 &&
                               fabs(z - 0.5) < 0.05);
         if (check) {
+          const double xh = hot_face ? x : 1.0 - x;  /* the coordinate in which the hot face is x = 1 */
           double a;
-          if (x == 1.0) a = 1;
-          else if (x == 0.0 || y == 0.0 || z == 0.0) a = 0;
+          if ((hot_face ? x == 1.0 : x == 0.0)) a = 1;
+          else if (x == 0.0 || x == 1.0 || y == 0.0 || z == 0.0) a = 0;
           else if (y == 1.0 || z == 1.0) a = 0;
-          else a = orc_soln(x, y, z, 300, 300);
+          else a = orc_soln(xh, y, z, 300, 300);
           double err = fabs(a - xv[roffset]);
           if (err > maxe) maxe = err;
         }

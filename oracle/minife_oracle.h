@@ -76,6 +76,13 @@ int orc_assemble_FE_data(const int gbox[6], int64_t nrows, const int* row_offset
  * z=0,y=1,z=1), which==1 -> bc_rows_1 (face x=1).  Sorted, unique.  Returns
  * the count; out may be NULL to query. */
 int64_t orc_bc_rows(const int gbox[6], int which, int* out);
+/* The same with the u = 1 face selectable: hot_face 1 = x=1 (reference),
+ * 0 = x=0 (the x=1 face then joins bc_rows_0).  Pinned by mirror symmetry. */
+int64_t orc_bc_rows_face(const int gbox[6], int which, int hot_face, int* out);
+/* assemble_FE_data with source Q (reference Q = 1; Q = 0: no source). */
+int orc_assemble_FE_data_q(const int gbox[6], int64_t nrows, const int* row_offsets,
+                           const int* cols, double* vals, double* b, const double* kappa,
+                           double Q);
 
 /* ref/src/SparseMatrix_functions.hpp:431-480 (+ zero_row_and_put_1 :411-428). */
 void orc_impose_dirichlet(double value, int64_t nrows, const int* row_offsets,
@@ -125,6 +132,9 @@ double orc_soln(double x, double y, double z, int max_p, int max_q);
 /* ref/fem/verify_solution.hpp:55-174; returns rc, *max_err the max abs error. */
 int orc_verify_solution(const int gbox[6], const double* x, double tolerance,
                         int verify_whole_domain, double* max_err);
+/* hot_face 0: the analytic solution mirrored, soln(1 - x, y, z). */
+int orc_verify_solution_face(const int gbox[6], const double* x, double tolerance,
+                             int verify_whole_domain, int hot_face, double* max_err);
 
 #ifdef __cplusplus
 }

@@ -77,3 +77,12 @@ def test_one_hip_runtime_per_process():
                              text=True, timeout=300)
         assert out.returncode == 0, out.stderr
         assert out.stdout.strip() == "1"
+
+
+@pytest.mark.parametrize("dims", [(3, 4, 5), (1, 1, 2)])
+def test_mesh_bc_sets_with_u1_on_x0(dims, oracle_mod):
+    from minife_amd import simple_mesh_description
+    nx, ny, nz = dims
+    m = simple_mesh_description([0, nx, 0, ny, 0, nz], hot_face=0)
+    assert np.array_equal(m.bc_rows_0, oracle_mod.bc_rows(nx, ny, nz, 0, hot_face=0))
+    assert np.array_equal(m.bc_rows_1, oracle_mod.bc_rows(nx, ny, nz, 1, hot_face=0))

@@ -57,3 +57,23 @@ def test_driver_eager_timers_and_mixed(tmp_path):
     assert r.returncode == 0, r.stderr
     assert float(doc["Mixed precision/Final relative residual"]) <= 1e-10
     assert int(doc["Mixed precision/Outer refinements"]) >= 1
+    sub2 = tmp_path / "mixed2"
+    sub2.mkdir()
+    r, doc = run_driver(sub2, "nx=20", "mixed=2", "verify_solution=1")
+    assert r.returncode == 0, r.stderr
+    assert float(doc["Mixed precision/Final relative residual"]) <= 1e-10
+
+
+def test_driver_config_text_variant(tmp_path, oracle_mod):
+    """bc_face=x0 source=0: the 30^3 CPU configuration as its text states it;
+    the replay solve equals the oracle's and verifies against the mirrored
+    analytic solution."""
+    r, doc = run_driver(tmp_path, "nx=30", "bc_face=x0", "source=0", "replay=1",
+                        "verify_solution=1")
+    assert r.returncode == 0, r.stderr
+    assert "solution matches analytic solution" in r.stdout
+    assert doc["Global Run Parameters/u=1 face"] == "x=0"
+    P = oracle_mod.build_problem(30, 30, 30, source=0.0, hot_face=0)
+    ref = oracle_mod.cg_solve(P["A"], P["b"], max_iter=200)
+    assert int(doc["CG solve/Iterations"]) == ref["num_iters"]
+    assert float(doc["CG solve/Final Resid Norm"]) == pytest.approx(ref["normr"], rel=1e-12)

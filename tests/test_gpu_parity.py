@@ -396,6 +396,27 @@ def test_mixed_keep_direction(mfe, ctx, dims):
     assert mfe.verify_solution(mesh, x)[0] == 0
 
 
+@pytest.mark.parametrize("dims", [(17, 3, 5), (30, 30, 30)])
+def test_config_text_variant_u1_on_x0_no_source(mfe, ctx, oracle_mod, dims):
+    """The configuration as its text states it (u = 1 on x = 0, zero source):
+    assembled operator and b bit-exact vs the oracle, replay CG bit-exact,
+    analytic check against the mirrored series."""
+    say(f"case x0/no-source {dims}")
+    mesh, A, b = mfe.build_problem(*dims, ctx=ctx, source=0.0, hot_face=0)
+    P = oracle_mod.build_problem(*dims, source=0.0, hot_face=0)
+    assert bits_equal(A.to_host(rows=False, row_offsets=False, cols=False)["vals"], P["A"].vals)
+    assert bits_equal(b.to_numpy(), P["b"])
+    x = mfe.Vector(A.nrows, ctx)
+    r = mfe.cg_solve(A, b, x, max_iter=200, replay=True)
+    ref = oracle_mod.cg_solve(P["A"], P["b"], max_iter=200)
+    assert r.num_iters == ref["num_iters"] and bits_equal(x.to_numpy(), ref["x"])
+    rc, err = mfe.verify_solution(mesh, x)
+    rco, erro = oracle_mod.verify_solution(*dims, ref["x"], hot_face=0)
+    assert rc == rco and err == erro
+    if dims == (30, 30, 30):
+        assert rc == 0
+
+
 def test_upload_rejects_bad_columns(mfe, ctx):
     with pytest.raises(mfe.MfeError) as e:
         mfe.CSRMatrix.from_host([0, 1, 2], [0, 5], [1.0, 1.0], ctx=ctx)

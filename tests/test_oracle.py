@@ -411,3 +411,35 @@ def test_openmp_cg_matches_serial(oracle_mod):
     assert np.max(np.abs(r4["x"] - r1["x"])) <= 1e-10 * np.max(np.abs(r1["x"]))
     r1t = oracle_mod.cg_solve_omp(P["A"], P["b"], max_iter=5, threads=1)
     assert np.array_equal(r1t["x"], oracle_mod.cg_solve(P["A"], P["b"], max_iter=5)["x"])
+
+
+def test_hot_face_mirror_symmetry(oracle_mod):
+    """u = 1 on x = 0 is the mirror image (x -> 1 - x) of the reference's u = 1
+    on x = 1: the two solutions agree node by node after mirroring (solver
+    tolerance; the element coordinates round differently)."""
+    dims = (7, 5, 6)
+    gx, gy, gz = 8, 6, 7
+    sols = []
+    for hot in (1, 0):
+        P = oracle_mod.build_problem(*dims, hot_face=hot)
+        bn = float(np.linalg.norm(P["b"]))
+        sols.append(oracle_mod.cg_solve(P["A"], P["b"], max_iter=2000, tol=1e-13 * bn)["x"])
+    x1 = sols[0].reshape(gz, gy, gx)
+    x0 = sols[1].reshape(gz, gy, gx)
+    assert np.max(np.abs(x0 - x1[:, :, ::-1])) <= 1e-10
+    b1 = oracle_mod.bc_rows(*dims, 1, hot_face=0)
+    assert np.all(b1 % gx == 0) and len(b1) == gy * gz
+
+
+def test_source_free_config_verifies(oracle_mod):
+    """The configuration as its text states it: u = 1 on x = 0, zero source,
+    30^3: no source entries, and the solution meets the analytic Laplace
+    solution closer than the reference's source-carrying default does."""
+    P = oracle_mod.build_problem(30, 30, 30, source=0.0, hot_face=0)
+    assert not np.any(P["b_prebc"])
+    r = oracle_mod.cg_solve(P["A"], P["b"], max_iter=200)
+    rc, err = oracle_mod.verify_solution(30, 30, 30, r["x"], hot_face=0)
+    Pd = oracle_mod.build_problem(30, 30, 30)
+    rd = oracle_mod.cg_solve(Pd["A"], Pd["b"], max_iter=200)
+    rcd, errd = oracle_mod.verify_solution(30, 30, 30, rd["x"])
+    assert rc == 0 and rcd == 0 and err < errd
