@@ -25,6 +25,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <fstream>
 #include <cstdio>
 #include <cstdlib>
 #include <limits>
@@ -305,6 +306,34 @@ void cg_solve(OperatorType& A, const VectorType& b, VectorType& x, Matvec /*matv
   normr = nr;
   if (my_cg_times)
     for (int i = 0; i < NUM_TIMERS; ++i) my_cg_times[i] = times[i];
+}
+
+// write_matrix (SparseMatrix_functions.hpp:98-140) and write_vector
+// (Vector_functions.hpp:49-82): the MINIFE_DEBUG dumps, one rank, to
+// "<filename>.1.0" -- "nrows nnz" then "row col coef" per entry (row-major,
+// stored order), or "local_size" then "index coef" -- with the default
+// stream formatting, so the files compare line for line with the reference's.
+template <typename MatrixType>
+void write_matrix(const std::string& filename, MatrixType& mat) {
+  std::vector<typename MatrixType::GlobalOrdinalType> rows, cols;
+  std::vector<typename MatrixType::LocalOrdinalType> ro;
+  std::vector<typename MatrixType::ScalarType> coefs;
+  mat.download(&rows, &ro, &cols, &coefs);
+  std::ofstream ofs(filename + ".1.0");
+  const size_t nrows = rows.size(), nnz = mat.num_nonzeros();
+  ofs << nrows << " " << nnz << std::endl;
+  for (size_t i = 0; i < nrows; ++i)
+    for (int j = ro[i]; j < ro[i + 1]; ++j)
+      ofs << rows[i] << " " << cols[(size_t)j] << " " << coefs[(size_t)j] << std::endl;
+}
+
+template <typename VectorType>
+void write_vector(const std::string& filename, const VectorType& vec) {
+  const std::vector<typename VectorType::ScalarType> coefs = vec.coefs();
+  std::ofstream ofs(filename + ".1.0");
+  ofs << vec.local_size << std::endl;
+  const typename VectorType::GlobalOrdinalType first = vec.startIndex;
+  for (size_t i = 0; i < (size_t)vec.local_size; ++i) ofs << first + i << " " << coefs[i] << std::endl;
 }
 
 // verify_solution.hpp:55-174; 0 when within tolerance.

@@ -10,7 +10,10 @@
 // trajectory), eager=1 (per-kernel CG timers), mixed=1 (FP32-inner / FP64
 // refinement solve to outer_tol, default 1e-10; mixed=2 keeps the FP32
 // search direction across refinements), bc_face=x0 (u = 1 on x = 0 instead
-// of x = 1), source=0 (no element source; the reference's Q is 1).
+// of x = 1), source=0 (no element source; the reference's Q is 1),
+// debug_dumps=1 (the MINIFE_DEBUG build's A_prebc/A/A_local.mtx.1.0,
+// b_prebc/b/b_local/x.vec.1.0 files and whole-domain verification,
+// driver.hpp:206-235,297-310).
 #include <sys/time.h>
 
 #include <cstdio>
@@ -116,6 +119,7 @@ int main(int argc, char** argv) {
   const int mixed = param(argc, argv, "mixed", 0);
   params.source = param(argc, argv, "source", 1.0);
   params.hot_face = arg_value(argc, argv, "bc_face") == "x0" ? 0 : 1;
+  const bool dumps = param(argc, argv, "debug_dumps", 0) != 0;
   const double outer_tol = param(argc, argv, "outer_tol", 1e-10);
 
   const double start_time = mytimer();
@@ -182,6 +186,10 @@ int main(int argc, char** argv) {
   doc.get("Matrix structure generation")->add("Mat-struc-gen Time", gen_structure);
   doc.add("FE assembly");
   doc.get("FE assembly")->add("FE assembly Time", fe_assembly);
+  if (dumps) {
+    write_matrix("A_prebc.mtx", A);
+    write_vector("b_prebc.vec", b);
+  }
 
   double dirbc_time = 0;
   RUN_TIMED_FUNCTION("imposing Dirichlet BC...",
@@ -192,9 +200,17 @@ int main(int argc, char** argv) {
                      impose_dirichlet(1.0, A, b, params.nx + 1, params.ny + 1, params.nz + 1,
                                       mesh.bc_rows_1),
                      dirbc_time, t_total);
+  if (dumps) {
+    write_matrix("A.mtx", A);
+    write_vector("b.vec", b);
+  }
   double make_local_time = 0;
   RUN_TIMED_FUNCTION("making matrix indices local...", make_local_matrix(A), make_local_time,
                      t_total);
+  if (dumps) {
+    write_matrix("A_local.mtx", A);
+    write_vector("b_local.vec", b);
+  }
 
   // compute_matrix_stats.hpp:80-108 (one rank)
   const size_t global_nnz = A.num_nonzeros();
@@ -247,14 +263,20 @@ int main(int argc, char** argv) {
               << inner << " FP32 iterations" << std::endl;
   }
   if (params.verify_solution > 0) {
-    std::cout << "verifying solution at ~ (0.5, 0.5, 0.5) ..." << std::endl;
+    const bool whole = dumps;  // the MINIFE_DEBUG build checks every node
+    if (whole)
+      std::cout << "verifying solution..." << std::endl;
+    else
+      std::cout << "verifying solution at ~ (0.5, 0.5, 0.5) ..." << std::endl;
     double err = 0;
-    verify_result = verify_solution(mesh, x, 0.06, false, &err);
+    verify_result = verify_solution(mesh, x, 0.06, whole, &err);
     if (verify_result)
       std::cout << "max absolute error is " << err << std::endl;
     else
       std::cout << "solution matches analytic solution to within 0.06 or better." << std::endl;
   }
+
+  if (dumps) write_vector("x.vec", x);
 
   Yaml* g = doc.get("Global Run Parameters");
   g->add("ScalarType", "double");

@@ -77,3 +77,31 @@ def test_driver_config_text_variant(tmp_path, oracle_mod):
     ref = oracle_mod.cg_solve(P["A"], P["b"], max_iter=200)
     assert int(doc["CG solve/Iterations"]) == ref["num_iters"]
     assert float(doc["CG solve/Final Resid Norm"]) == pytest.approx(ref["normr"], rel=1e-5)
+
+
+DUMPS = os.path.join(ROOT, "tests", "golden", "dumps")
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 2), (2, 1, 1), (17, 3, 5), (20, 30, 40)])
+def test_driver_debug_dumps_match_reference(tmp_path, dims):
+    """The reference CI's parity check (diff of the MINIFE_DEBUG dumps between
+    implementations, one rank): debug_dumps=1 replay=1 writes A_prebc, A,
+    A_local, b_prebc, b, b_local and x files identical to the reference
+    miniFE.x's (gzipped fixtures; SHA-256 at 20x30x40)."""
+    import gzip
+    import hashlib
+    import json
+    nx, ny, nz = dims
+    r, _ = run_driver(tmp_path, f"nx={nx}", f"ny={ny}", f"nz={nz}", "debug_dumps=1", "replay=1")
+    assert r.returncode == 0, r.stderr
+    case = "x".join(map(str, dims))
+    names = ["A_prebc.mtx.1.0", "A.mtx.1.0", "A_local.mtx.1.0", "b_prebc.vec.1.0", "b.vec.1.0",
+             "b_local.vec.1.0", "x.vec.1.0"]
+    big = os.path.join(DUMPS, case + ".json")
+    for f in names:
+        got = open(tmp_path / f, "rb").read()
+        if os.path.exists(big):
+            assert hashlib.sha256(got).hexdigest() == json.load(open(big))[f], f
+        else:
+            want = gzip.open(os.path.join(DUMPS, case, f + ".gz"), "rb").read()
+            assert got == want, f
