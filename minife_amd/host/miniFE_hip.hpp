@@ -23,9 +23,11 @@
 //   verify_solution           fem/verify_solution.hpp:55-174
 #pragma once
 
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <fstream>
+#include <iostream>
 #include <cstdio>
 #include <cstdlib>
 #include <limits>
@@ -40,6 +42,11 @@
 namespace miniFE {
 
 typedef double timer_type;
+// Wall clock in seconds (the role of mytimer(), ref/utils/mytimer.cpp).
+inline double host_seconds() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
 enum CG_TIMES { WAXPY = 0, DOT = 1, MATVEC = 2, MATVECDOT = 3, TOTAL = 4, NUM_TIMERS = 5 };
 
 // Runtime options of the HIP path that have no reference counterpart.
@@ -306,6 +313,41 @@ void cg_solve(OperatorType& A, const VectorType& b, VectorType& x, Matvec /*matv
   normr = nr;
   if (my_cg_times)
     for (int i = 0; i < NUM_TIMERS; ++i) my_cg_times[i] = times[i];
+}
+
+// time_kernels (time_kernels.hpp:42-134): max_iter back-to-back waxpby,
+// matvec and dot calls, each group timed; xdotp = the sum of the dots.
+template <typename OperatorType, typename VectorType, typename Matvec>
+void time_kernels(OperatorType& A, const VectorType& b, VectorType& x, Matvec matvec,
+                  typename OperatorType::LocalOrdinalType max_iter,
+                  typename OperatorType::ScalarType& xdotp, timer_type* my_kern_times) {
+  if (!A.has_local_indices) {
+    std::cerr << "miniFE::time_kernels ERROR, A.has_local_indices is false, needs to be true. "
+                 "This probably means miniFE::make_local_matrix(A) was not called prior to "
+                 "calling miniFE::time_kernels."
+              << std::endl;
+    return;
+  }
+  (void)b;
+  VectorType p(0, (typename VectorType::LocalOrdinalType)A.num_rows());
+  const double one = 1.0, zero = 0.0;
+  check(mfe_ctx_synchronize(context()), "time_kernels");
+  double t0 = host_seconds();
+  for (int i = 0; i < max_iter; ++i) waxpby(one, x, zero, x, p);
+  check(mfe_ctx_synchronize(context()), "time_kernels");
+  const double tW = host_seconds() - t0;
+  t0 = host_seconds();
+  for (int i = 0; i < max_iter; ++i) matvec(A, p, x);
+  check(mfe_ctx_synchronize(context()), "time_kernels");
+  const double tM = host_seconds() - t0;
+  t0 = host_seconds();
+  xdotp = 0;
+  for (int i = 0; i < max_iter; ++i) xdotp += dot(x, p);
+  const double tD = host_seconds() - t0;
+  my_kern_times[WAXPY] = tW;
+  my_kern_times[DOT] = tD;
+  my_kern_times[MATVEC] = tM;
+  my_kern_times[TOTAL] = 0;
 }
 
 // write_matrix (SparseMatrix_functions.hpp:98-140) and write_vector

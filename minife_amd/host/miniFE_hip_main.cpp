@@ -13,7 +13,8 @@
 // of x = 1), source=0 (no element source; the reference's Q is 1),
 // debug_dumps=1 (the MINIFE_DEBUG build's A_prebc/A/A_local.mtx.1.0,
 // b_prebc/b/b_local/x.vec.1.0 files and whole-domain verification,
-// driver.hpp:206-235,297-310).
+// driver.hpp:206-235,297-310), time_kernels=1 (the MINIFE_KERNELS build:
+// 500 timed waxpby, matvec and dot calls, time_kernels.hpp:42-134).
 #include <sys/time.h>
 
 #include <cstdio>
@@ -120,6 +121,7 @@ int main(int argc, char** argv) {
   params.source = param(argc, argv, "source", 1.0);
   params.hot_face = arg_value(argc, argv, "bc_face") == "x0" ? 0 : 1;
   const bool dumps = param(argc, argv, "debug_dumps", 0) != 0;
+  const bool kernels = param(argc, argv, "time_kernels", 0) != 0;
   const double outer_tol = param(argc, argv, "outer_tol", 1e-10);
 
   const double start_time = mytimer();
@@ -237,7 +239,16 @@ int main(int argc, char** argv) {
   double cg_times[NUM_TIMERS] = {0, 0, 0, 0, 0};
   int verify_result = 0;
   std::string title("CG solve");
-  if (!mixed) {
+  if (kernels) {  // the MINIFE_KERNELS build (driver.hpp:257-272)
+    std::cout.width(30);
+    std::cout << "Starting kernel timing loops ..." << std::endl;
+    num_iters = 500;
+    std::vector<double> x0((size_t)local_nrows, 0.0);
+    if (local_nrows > 0) x0[0] = 0.9;
+    x.assign(x0);
+    time_kernels(A, b, x, matvec_std<MatrixType, VectorType>(), num_iters, rnorm, cg_times);
+    title = "Kernel timings";
+  } else if (!mixed) {
     std::cout << "Starting CG solver ... " << std::endl;
     cg_solve(A, b, x, matvec_std<MatrixType, VectorType>(), max_iters, tol, num_iters, rnorm,
              cg_times);
@@ -288,9 +299,10 @@ int main(int argc, char** argv) {
   // driver.hpp:330-345, including its quirk of counting dot and waxpy flops
   // over nx*ny*nz (elements) rather than rows.
   const double global_nrows = (double)params.nx * params.ny * params.nz;
-  double mv_flops = global_nnz * 2.0 * (num_iters + 1);
-  double dot_flops = global_nrows * 2.0 * (2 * num_iters);
-  double waxpy_flops = global_nrows * 3.0 * (3 * num_iters + 2);
+  // (the kernel-timing build counts one of each operation per iteration)
+  double mv_flops = global_nnz * 2.0 * (kernels ? num_iters : num_iters + 1);
+  double dot_flops = global_nrows * 2.0 * (kernels ? num_iters : 2 * num_iters);
+  double waxpy_flops = global_nrows * 3.0 * (kernels ? num_iters : 3 * num_iters + 2);
   const double total_flops = mv_flops + dot_flops + waxpy_flops;
   auto mflops = [](double f, double t) { return t > 1.e-4 ? 1.e-6 * (f / t) : -1.0; };
   auto add_rate = [&](const char* k, double f, double t) {
@@ -307,13 +319,15 @@ int main(int argc, char** argv) {
   c->add("MATVEC Time", cg_times[MATVEC]);
   c->add("MATVEC Flops", mv_flops);
   add_rate("MATVEC Mflops", mv_flops, cg_times[MATVEC]);
-  Yaml* tot = c->add("Total");
-  tot->add("Total CG Time", cg_times[TOTAL]);
-  tot->add("Total CG Flops", total_flops);
-  const double tm = mflops(total_flops, cg_times[TOTAL]);
-  if (tm >= 0) tot->add("Total CG Mflops", tm);
-  else tot->add("Total CG Mflops", "inf");
-  c->add("Time per iteration", num_iters > 0 ? cg_times[TOTAL] / num_iters : 0.0);
+  if (!kernels) {
+    Yaml* tot = c->add("Total");
+    tot->add("Total CG Time", cg_times[TOTAL]);
+    tot->add("Total CG Flops", total_flops);
+    const double tm = mflops(total_flops, cg_times[TOTAL]);
+    if (tm >= 0) tot->add("Total CG Mflops", tm);
+    else tot->add("Total CG Mflops", "inf");
+    c->add("Time per iteration", num_iters > 0 ? cg_times[TOTAL] / num_iters : 0.0);
+  }
   doc.add("Total Program Time", mytimer() - start_time);
 
   // YAML_Doc.cpp:32-72 file naming

@@ -105,3 +105,22 @@ def test_driver_debug_dumps_match_reference(tmp_path, dims):
         else:
             want = gzip.open(os.path.join(DUMPS, case, f + ".gz"), "rb").read()
             assert got == want, f
+
+
+def test_driver_time_kernels_mode(tmp_path, oracle_mod):
+    """time_kernels=1 (the reference's MINIFE_KERNELS build): 500 each of
+    waxpby, matvec, dot from x = (0.9, 0, ...); 'Final Resid Norm' carries the
+    summed dots, which equal 500 * (A p).p of the oracle."""
+    r, doc = run_driver(tmp_path, "nx=10", "time_kernels=1")
+    assert r.returncode == 0, r.stderr
+    assert int(doc["Kernel timings/Iterations"]) == 500
+    for k in ("WAXPY", "DOT", "MATVEC"):
+        assert float(doc[f"Kernel timings/{k} Time"]) > 0
+    P = oracle_mod.build_problem(10, 10, 10)
+    p = np.zeros(P["A"].nrows)
+    p[0] = 0.9
+    ap = oracle_mod.matvec(P["A"], p)
+    assert float(doc["Kernel timings/Final Resid Norm"]) == pytest.approx(500 * float(ap @ p),
+                                                                          rel=1e-5)
+    nnz = P["A"].row_offsets[-1]
+    assert float(doc["Kernel timings/MATVEC Flops"]) == pytest.approx(2.0 * nnz * 500, rel=1e-5)
