@@ -810,7 +810,9 @@ __global__ __launch_bounds__(kBlock) void cg_p_kernel(int64_t n, const CgState* 
     const T a = (T)st->alpha_prev;
     for (int64_t i = i0; i < n; i += stride) {
       const T pi = p[i];
-      x[i] = T(1) * x[i] + a * pi;
+      // x is not read again until the next iteration's cg_p: stream it past
+      // the caches the SpMV's gathers of p are about to use
+      __builtin_nontemporal_store(T(1) * x[i] + a * pi, x + i);
       p[i] = gamma * r[i] + beta * (y == p ? pi : y[i]);
     }
   } else {
