@@ -48,6 +48,30 @@ def sell_bytes(n: int, plan: dict, value_bytes: int = 8) -> int:
     return (value_bytes + 4) * slots + 64 * ns + 4 * (ns + 1) + 2 * value_bytes * n
 
 
+def sellx_bytes(n: int, plan: dict, value_bytes: int = 8) -> int:
+    """HBM bytes of one SELL-64 SpMV with staged x: every stored entry as value
+    + 16-bit local column, one length byte per row slot, the slice offsets,
+    the plan's runs (first column, list position) and per-tile run offsets and
+    distinct counts, x read once and y written once."""
+    slots, ns, runs = plan["stored"], plan["slices"], plan["runs"]
+    tiles = -(-n // 256)
+    return ((value_bytes + 2) * slots + 64 * ns + 4 * (ns + 1) + 8 * runs + 4 * (tiles + 1)
+            + 4 * tiles + 2 * value_bytes * n)
+
+
+def plan_bytes(n: int, nnz: int, plan: dict, value_bytes: int = 8) -> int:
+    """Bytes of one SpMV in the format the plan chose."""
+    if plan["kernel"] == "sell64-staged-x":
+        return sellx_bytes(n, plan, value_bytes)
+    if plan["kernel"] == "sell64":
+        return sell_bytes(n, plan, value_bytes)
+    return spmv_bytes(n, nnz, value_bytes)
+
+
+KERNEL_NAMES = {"sell64-staged-x": "spmv_sellx_kernel", "sell64": "spmv_sell_kernel",
+                "csr-stream": "spmv_stream_kernel", "csr-rowwise": "spmv_rowwise_kernel"}
+
+
 def measured_traffic(N: int, kernel: str):
     """HBM bytes per launch of the CG SpMV kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary for this workload (calibrated against a
@@ -164,7 +188,10 @@ def bench_multi_rhs(mfe, ctx):
         t_single += time.perf_counter() - t0
         its_single += rk.num_iters
     ms = tun["ms"][tun["tile"]]
+    # CSR-equivalent bytes of one SpMM (values, columns, row offsets, X and Y once)
     spmm_bytes = 12 * nnz + 4 * (n + 1) + 2 * 8 * K * n
+    direct = [v for v, t in zip(tun["ms"], tun["tiles"]) if not t["staged"]]
+    staged = [v for v, t in zip(tun["ms"], tun["tiles"]) if t["staged"] and np.isfinite(v)]
     return {"workload": "128^3, 16 RHS N(0,1) per free node seed 1, FP64 CG to 1e-10 per column",
             "seconds": r.seconds, "iters_min": int(r.iters.min()), "iters_max": int(r.iters.max()),
             "max_rel_residual": float(r.rel_residual.max()),
@@ -172,7 +199,8 @@ def bench_multi_rhs(mfe, ctx):
             "speedup_vs_single": t_single / r.seconds,
             "spmm_tile": tun["tiles"][tun["tile"]], "spmm_ms": ms,
             "spmm_GBs": spmm_bytes / (ms * 1e-3) / 1e9,
-            "tuner_ms": [round(v, 4) for v in tun["ms"]]}
+            "best_direct_ms": min(direct), "best_staged_ms": min(staged) if staged else None,
+            "tuner_ms": [round(v, 4) if np.isfinite(v) else None for v in tun["ms"]]}
 
 
 def bench_adjoint(mfe):
@@ -282,14 +310,24 @@ def main():
     ms_32 = mfe.time_matvec(A, xs, ys, variant=2, reps=args.spmv_reps)
     ms_csr = mfe.time_matvec(A, xs, ys, variant=3, reps=args.spmv_reps)
     ms_csr32 = mfe.time_matvec(A, xs, ys, variant=4, reps=args.spmv_reps)
-    sell = plan["kernel"] == "sell64"
-    byt = sell_bytes(n, plan) if sell else spmv_bytes(n, nnz)
-    byt32 = sell_bytes(n, plan, 4) if sell else spmv_bytes(n, nnz, 4)
+    staged = plan["kernel"] == "sell64-staged-x"
+    ms_sell = mfe.time_matvec(A, xs, ys, variant=6, reps=args.spmv_reps) if staged else None
+    byt = plan_bytes(n, nnz, plan)
+    byt32 = plan_bytes(n, nnz, plan, 4)
+    kname = KERNEL_NAMES[plan["kernel"]]
     gbs_cg = byt / (ms_cg * 1e-3) / 1e9
     gbs_std = byt / (ms_std * 1e-3) / 1e9
     gbs_32 = byt32 / (ms_32 * 1e-3) / 1e9
     gbs_csr = spmv_bytes(n, nnz) / (ms_csr * 1e-3) / 1e9
     gbs_csr32 = spmv_bytes(n, nnz, 4) / (ms_csr32 * 1e-3) / 1e9
+    sell_cmp = None
+    if staged:
+        sb = sell_bytes(n, plan)
+        sell_cmp = {"fp64_ms": ms_sell, "fp64_GBs": sb / (ms_sell * 1e-3) / 1e9,
+                    "bytes_per_launch": sb,
+                    "staged_x_speedup": ms_sell / ms_cg}
+        log(f"plain SELL-64 (no staged x) {ms_sell:.4f} ms -> {sell_cmp['fp64_GBs']:.1f} GB/s; "
+            f"staged x is {ms_sell / ms_cg:.3f}x faster")
     log(f"spmv ({plan['kernel']}, cg kernel) {ms_cg:.4f} ms -> {gbs_cg:.1f} GB/s; matvec_std "
         f"{ms_std:.4f} ms -> {gbs_std:.1f} GB/s; fp32 {ms_32:.4f} ms -> {gbs_32:.1f} GB/s; "
         f"csr kernels: fp64 {ms_csr:.4f} ms ({gbs_csr:.1f} GB/s), fp32 {ms_csr32:.4f} ms "
@@ -387,11 +425,9 @@ def main():
                    "nx": N, "ny": N, "nz": N, "nrows": n, "nnz": nnz,
                    "cg_iters_per_step": args.iters},
         "roofline": {"bound": "hbm", "achieved": gbs_cg, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs_cg / HBM_PEAK_GBS, "traffic": measured_traffic(
-                         N, "spmv_sell_kernel<double" if sell else "spmv_stream_kernel<double"),
-                     "kernel": ("spmv_sell_kernel<double,false,kDot>" if sell else
-                                "spmv_stream_kernel<double,false,kDot>")
-                               + " (CG SpMV + p.Ap partials)",
+                     "frac": gbs_cg / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(N, kname + "<double"),
+                     "kernel": kname + "<double,false,kDot> (CG SpMV + p.Ap partials)",
                      "bytes_per_launch": byt, "avg_ms": ms_cg},
         "spmv_GBs": gbs_cg,
         "spmv_std_GBs": gbs_std,
@@ -399,6 +435,7 @@ def main():
         "spmv_plan": dict(plan, build_s=t_plan),
         "spmv_csr_kernels": {"fp64_ms": ms_csr, "fp64_GBs": gbs_csr, "fp32_ms": ms_csr32,
                              "fp32_GBs": gbs_csr32},
+        "spmv_sell_kernel": sell_cmp,
         "assembly": {"struct_s": t_struct, "fe_s": t_asm, "dirichlet_s": t_bc,
                      "elements_per_s": ne / t_asm},
         "time_to_1e-10": tol_run,

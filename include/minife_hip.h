@@ -138,17 +138,24 @@ int mfe_make_local_matrix(mfe_ctx* ctx, mfe_csr* A);
  * CSR loop).  MFE_SPMV_AUTO (default) keeps a SELL-64 copy of A beside the CSR
  * arrays -- slices of 64 rows, one row per lane, slot j of a slice stored as
  * 64 consecutive values/columns -- built on first use and rebuilt after the
- * values or structure change; matrices with a row longer than 27 entries, or
- * whose copy does not fit in device memory, stay on the CSR kernels.
- * MFE_SPMV_CSR frees the copy and uses the CSR kernels only.  Every format
- * sums each row in stored column order, so results are bit-identical. */
+ * values or structure change -- and, per 256-row tile, a staged-x plan: the
+ * tile's distinct columns as runs of consecutive columns, with 16-bit local
+ * columns per slot, so the SpMV copies the tile's x entries into LDS once and
+ * gathers from there.  A tile with more than 128 runs leaves the matrix on
+ * plain SELL-64; a row longer than 27 entries, or a copy that does not fit in
+ * device memory, on the CSR kernels.  MFE_SPMV_SELL: SELL-64 without the
+ * staged x.  MFE_SPMV_CSR frees the copies and uses the CSR kernels only.
+ * Every format sums each row in stored column order, so results are
+ * bit-identical. */
 #define MFE_SPMV_AUTO 0
 #define MFE_SPMV_CSR 1
+#define MFE_SPMV_SELL 2
 /* Sets A's format and builds its FP64 plan now (so later solves do not pay
  * for it).  info (nullable) receives {kernel in use: 0 CSR row-stream,
- * 1 SELL-64, 2 CSR row-per-thread; stored entries incl. padding; slices;
- * longest row}. */
-int mfe_spmv_plan(mfe_ctx* ctx, mfe_csr* A, int format, int64_t info[4]);
+ * 1 SELL-64, 2 CSR row-per-thread, 3 SELL-64 with staged x; stored entries
+ * incl. padding; slices; longest row; runs of the staged-x plan (0 if not in
+ * use); most distinct columns of a 256-row tile (0 if not in use)}. */
+int mfe_spmv_plan(mfe_ctx* ctx, mfe_csr* A, int format, int64_t info[6]);
 /* matvec_std<CSRMatrix>::operator()(A, x, y), SparseMatrix_functions.hpp:545-578.
  * Each row's products are summed in stored column order: bit-identical. */
 int mfe_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y);
@@ -232,7 +239,8 @@ int mfe_cg_solve_mixed_ex(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* b, mfe_
  * the FP32 kernel inside mfe_cg_solve_mixed's inner CG (x is used as its
  * FP32 copy, converted before the timed launches).  Variants 3 and 4 are 1
  * and 2 on the CSR kernels whatever A's plan (for comparison); variant 5 is
- * 1 reading the CG's (clear) done flag first, as it does inside the solve. */
+ * 1 reading the CG's (clear) done flag first, as it does inside the solve;
+ * variant 6 is 1 on plain SELL-64 where A's plan stages x (for comparison). */
 int mfe_time_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y, int variant,
                     int reps, double* avg_ms);
 
@@ -280,9 +288,18 @@ int mfe_spmm(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* X, mfe_vec* Y, int n
 /* Autotuner: times every candidate tile on A (3 launches each, HIP events),
  * keeps the fastest for A and nrhs; tile, ms[ntiles] and ntiles nullable. */
 int mfe_spmm_tune(mfe_ctx* ctx, const mfe_csr* A, int nrhs, int* tile, double* ms, int* ntiles);
+/* Pins `tile` as the tuned tile of A for nrhs (mfe_spmm with tile -1 and
+ * mfe_cg_solve_multi use it) without timing; fails for an unavailable tile.
+ * Kept until A's structure changes or another matrix is tuned on ctx. */
+int mfe_spmm_select(mfe_ctx* ctx, const mfe_csr* A, int nrhs, int tile);
 /* Shape of a tile: columns per lane, non-zeros per unrolled chunk, rows per
- * 256-thread workgroup. */
-int mfe_spmm_tile_info(int nrhs, int tile, int* cols_per_lane, int* unroll, int* rows_per_block);
+ * workgroup, and staged = 1 for tiles that copy the distinct X rows of 32
+ * consecutive rows into LDS before the products (available when no 32-row
+ * tile has more than 1024 non-zeros and its X rows fit 64 KiB; the tuner
+ * reports ms = +inf for an unavailable tile, and mfe_spmm rejects it).
+ * Outputs nullable. */
+int mfe_spmm_tile_info(int nrhs, int tile, int* cols_per_lane, int* unroll, int* rows_per_block,
+                       int* staged);
 /* nrhs simultaneous FP64 CG solves A X[:, k] = B[:, k] from the given X,
  * sharing one SpMM per iteration; column k stops once ||r_k|| <= rel_tol *
  * ||B[:, k]|| (cg_solve.hpp's loop test per column) or after max_iter.

@@ -34,8 +34,8 @@ LIB_PATH = os.path.join(_HERE, "libminife_hip.so")
 MFE_OK, MFE_BREAKDOWN, MFE_NOT_LOCAL, MFE_OOM, MFE_HIP_ERROR, MFE_BAD_ARG, MFE_UNSUPPORTED = range(7)
 STATUS_NAMES = ["OK", "BREAKDOWN", "NOT_LOCAL", "OOM", "HIP_ERROR", "BAD_ARG", "UNSUPPORTED"]
 CG_REPLAY, CG_TOL_RELATIVE, CG_EAGER = 0x1, 0x2, 0x4
-SPMV_AUTO, SPMV_CSR = 0, 1
-SPMV_KERNELS = ["csr-stream", "sell64", "csr-rowwise"]
+SPMV_AUTO, SPMV_CSR, SPMV_SELL = 0, 1, 2
+SPMV_KERNELS = ["csr-stream", "sell64", "csr-rowwise", "sell64-staged-x"]
 T_WAXPY, T_DOT, T_MATVEC, T_MATVECDOT, T_TOTAL = range(5)
 
 # Every extern "C" symbol declared in include/minife_hip.h.
@@ -51,7 +51,7 @@ EXPORTS = [
     "mfe_matvec", "mfe_matvec_and_dot", "mfe_dot", "mfe_dot_replay", "mfe_waxpby",
     "mfe_cg_solve", "mfe_cg_solve_mixed", "mfe_cg_solve_mixed_ex", "mfe_matvec_fp32", "mfe_time_matvec", "mfe_soln",
     "mfe_verify_solution", "mfe_batch_create", "mfe_batch_destroy", "mfe_batch_info",
-    "mfe_batch_solve", "mfe_batch_download", "mfe_batch_timing", "mfe_spmm", "mfe_spmm_tune", "mfe_spmm_tile_info",
+    "mfe_batch_solve", "mfe_batch_download", "mfe_batch_timing", "mfe_spmm", "mfe_spmm_tune", "mfe_spmm_select", "mfe_spmm_tile_info",
     "mfe_cg_solve_multi",
 ]
 
@@ -169,8 +169,10 @@ def lib() -> ctypes.CDLL:
         "mfe_spmm": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "mfe_spmm_tune": ([_vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), _vp,
                            ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "mfe_spmm_select": ([_vp, _vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
         "mfe_spmm_tile_info": ([ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
-                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)],
+                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                ctypes.POINTER(ctypes.c_int)],
                                ctypes.c_int),
         "mfe_cg_solve_multi": ([_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                 _vp, _vp, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
@@ -460,12 +462,14 @@ def make_local_matrix(A: CSRMatrix) -> None:
 
 
 def spmv_plan(A: CSRMatrix, fmt: int = SPMV_AUTO) -> dict:
-    """Choose A's SpMV storage (SPMV_AUTO: SELL-64 copy when every row has at
-    most 27 entries; SPMV_CSR: CSR kernels only) and build it now."""
-    info = (ctypes.c_int64 * 4)()
+    """Choose A's SpMV storage and build it now.  SPMV_AUTO: SELL-64 copy when
+    every row has at most 27 entries, with x staged in LDS per 256-row tile
+    when every tile's distinct columns form at most 128 runs; SPMV_SELL:
+    SELL-64 without the staging; SPMV_CSR: CSR kernels only."""
+    info = (ctypes.c_int64 * 6)()
     A.ctx.check(lib().mfe_spmv_plan(A.ctx.h, A.h, int(fmt), info), "spmv_plan")
     return dict(kernel=SPMV_KERNELS[info[0]], stored=int(info[1]), slices=int(info[2]),
-                longest_row=int(info[3]))
+                longest_row=int(info[3]), runs=int(info[4]), max_distinct=int(info[5]))
 
 
 def matvec(A: CSRMatrix, x: Vector, y: Vector) -> None:
@@ -571,7 +575,8 @@ def time_matvec(A: CSRMatrix, x: Vector, y: Vector, variant: int = 1, reps: int 
     """Mean SpMV kernel time in ms over `reps` back-to-back launches (HIP events
     on the context's stream); variant 1 is the kernel inside the fast cg_solve,
     variant 2 the FP32 kernel inside cg_solve_mixed, 3 and 4 the same two on
-    the CSR kernels whatever A's plan."""
+    the CSR kernels whatever A's plan, 6 variant 1 on plain SELL-64 where the
+    plan stages x."""
     ms = ctypes.c_double(0)
     A.ctx.check(lib().mfe_time_matvec(A.ctx.h, A.h, x.h, y.h, int(variant), int(reps),
                                       ctypes.byref(ms)), "time_matvec")
@@ -592,17 +597,34 @@ def spmm(A: CSRMatrix, X: Vector, Y: Vector, nrhs: int, tile: int = -1) -> None:
 
 
 def spmm_tune(A: CSRMatrix, nrhs: int) -> dict:
-    """Run the SpMM autotuner on A: {'tile': chosen id, 'ms': [...], 'tiles': [...]}."""
+    """Run the SpMM autotuner on A: {'tile': chosen id, 'ms': [...], 'tiles': [...]};
+    ms is +inf for a tile that cannot run on A (a staged tile whose plan does
+    not fit)."""
     ms = np.zeros(16, np.float64)
     tile, nt = ctypes.c_int(0), ctypes.c_int(0)
     A.ctx.check(lib().mfe_spmm_tune(A.ctx.h, A.h, int(nrhs), ctypes.byref(tile), _ptr(ms),
                                     ctypes.byref(nt)), "spmm_tune")
+    return {"tile": tile.value, "ms": ms[:nt.value].tolist(), "tiles": spmm_tiles(nrhs)}
+
+
+def spmm_select(A: CSRMatrix, nrhs: int, tile: int) -> None:
+    """Pin `tile` as A's SpMM tile for nrhs (used by spmm(tile=-1) and
+    cg_solve_multi) without timing."""
+    A.ctx.check(lib().mfe_spmm_select(A.ctx.h, A.h, int(nrhs), int(tile)), "spmm_select")
+
+
+def spmm_tiles(nrhs: int) -> list:
+    """Shapes of the SpMM candidate tiles for nrhs (index = tile id)."""
     tiles = []
-    for i in range(nt.value):
-        c, u, r = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
-        lib().mfe_spmm_tile_info(int(nrhs), i, ctypes.byref(c), ctypes.byref(u), ctypes.byref(r))
-        tiles.append({"cols_per_lane": c.value, "unroll": u.value, "rows_per_block": r.value})
-    return {"tile": tile.value, "ms": ms[:nt.value].tolist(), "tiles": tiles}
+    i = 0
+    while True:
+        c, u, r, st = (ctypes.c_int(0) for _ in range(4))
+        if lib().mfe_spmm_tile_info(int(nrhs), i, ctypes.byref(c), ctypes.byref(u),
+                                    ctypes.byref(r), ctypes.byref(st)) != 0:
+            return tiles
+        tiles.append({"cols_per_lane": c.value, "unroll": u.value, "rows_per_block": r.value,
+                      "staged": bool(st.value)})
+        i += 1
 
 
 @dataclass

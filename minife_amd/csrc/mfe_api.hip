@@ -156,6 +156,7 @@ int mfe_csr_create(mfe_ctx* ctx, mfe_csr** out) {
   if (!ctx || !out) return MFE_BAD_ARG;
   *out = new mfe_csr();
   (*out)->ctx = ctx;
+  (*out)->plan_gen = next_plan_gen();
   return MFE_OK;
 }
 

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -58,7 +59,10 @@ struct mfe_csr {
   // rows, one row per lane, slot j of a slice stored as 64 consecutive
   // entries), derived lazily from the CSR arrays and rebuilt after changes.
   int spmv_format = MFE_SPMV_AUTO;
-  uint64_t plan_gen = 0;  // bumped whenever the arrays an SpMV launch reads move
+  // Process-unique stamp, renewed whenever the arrays an SpMV launch reads
+  // move: (matrix, plan_gen) names one structure even across a matrix freed
+  // and another created at its address.
+  uint64_t plan_gen = 0;
   bool sell_struct_valid = false;  // sell_off/len/cols match the CSR structure
   bool sell_ok = false;            // every row fits kSellMaxWidth and the copy fits memory
   bool sell_vals_valid = false, sell_vals32_valid = false;
@@ -69,6 +73,18 @@ struct mfe_csr {
   int* sell_cols = nullptr;
   double* sell_vals = nullptr;
   float* sell_vals32 = nullptr;
+  // SELL-64 with the x entries of each 256-row tile staged in LDS: the tile's
+  // distinct columns as runs of consecutive columns, and every slot's column
+  // as its 16-bit position in the tile's list (same slot layout as sell_cols).
+  bool sellx_valid = false;  // examined for the current SELL structure
+  bool sellx_ok = false;     // every tile fits (runs, LDS) and the copy fits memory
+  int sellx_maxu = 0;        // most distinct columns of any tile
+  int64_t sellx_ntiles = 0, sellx_runs = 0;
+  unsigned short* sellx_lcols = nullptr;  // sell_slots + tail
+  int* sellx_roff = nullptr;              // ntiles+1: first run of each tile
+  int* sellx_rcol = nullptr;              // runs: first column of the run
+  int* sellx_rpos = nullptr;              // runs: its position in the tile's list
+  int* sellx_nu = nullptr;                // ntiles: distinct columns of the tile
 };
 
 struct mfe_vec {
@@ -87,6 +103,11 @@ int plan_spmv(mfe_ctx* ctx, mfe_csr* A);
 
 __host__ __device__ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+inline uint64_t next_plan_gen() {
+  static std::atomic<uint64_t> g{0};
+  return ++g;
+}
+
 // Derived copies (FP32 values, SELL plan) are stale after these.
 inline void csr_values_changed(mfe_csr* A) {
   A->vals32_valid = false;
@@ -95,7 +116,7 @@ inline void csr_values_changed(mfe_csr* A) {
 }
 inline void csr_structure_changed(mfe_csr* A) {
   csr_values_changed(A);
-  ++A->plan_gen;
+  A->plan_gen = next_plan_gen();
   A->max_block_nnz = -1;
   A->sell_struct_valid = false;
   A->sell_ok = false;

@@ -9,8 +9,10 @@
 // value/column entry (a broadcast).  The row is walked in chunks of U entries
 // whose loads are all issued before the first product; each column's sum runs
 // in stored column order, so column k of Y equals A * X[:, k] from the
-// single-vector SpMV bit for bit.  The autotuner times every (CPL, U) tile on
-// the caller's matrix and keeps the fastest for that matrix and K.
+// single-vector SpMV bit for bit.  Staged tiles first copy the distinct X rows
+// of 32 consecutive rows into LDS and gather from there (same arithmetic).
+// The autotuner times every tile on the caller's matrix and keeps the fastest
+// for that matrix and K.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -21,13 +23,18 @@ namespace mfe {
 
 struct SpmmTile {
   int cpl, unroll;
+  int rows;  // 0: direct gathers; 16 or 32: rows per workgroup of a staged tile
 };
 
 // Candidate tiles per K (index = tile id).
-static const SpmmTile kTiles16[] = {{1, 9}, {2, 9}, {4, 9}, {8, 9},
-                                    {2, 3}, {4, 3}, {8, 3}, {16, 3}};
-static const SpmmTile kTiles8[] = {{1, 9}, {2, 9}, {4, 9}, {8, 9}, {2, 3}, {4, 3}, {8, 3}};
-static const SpmmTile kTiles4[] = {{1, 9}, {2, 9}, {4, 9}, {2, 3}, {4, 3}};
+static const SpmmTile kTiles16[] = {{1, 9, 0},  {2, 9, 0},  {4, 9, 0},  {8, 9, 0},  {2, 3, 0},
+                                    {4, 3, 0},  {8, 3, 0},  {16, 3, 0}, {1, 9, 16}, {2, 9, 16},
+                                    {1, 9, 32}, {2, 9, 32}, {4, 9, 32}};
+static const SpmmTile kTiles8[] = {{1, 9, 0},  {2, 9, 0},  {4, 9, 0},  {8, 9, 0},  {2, 3, 0},
+                                   {4, 3, 0},  {8, 3, 0},  {1, 9, 16}, {1, 9, 32}, {2, 9, 32}};
+static const SpmmTile kTiles4[] = {{1, 9, 0}, {2, 9, 0},  {4, 9, 0},
+                                   {2, 3, 0}, {4, 3, 0},  {1, 9, 16}, {1, 9, 32}};
+constexpr int kMaxTiles = 16;
 
 static int tile_count(int K) {
   return K == 16 ? (int)(sizeof(kTiles16) / sizeof(SpmmTile))
@@ -36,6 +43,119 @@ static int tile_count(int K) {
 }
 static SpmmTile tile_of(int K, int id) {
   return K == 16 ? kTiles16[id] : K == 8 ? kTiles8[id] : kTiles4[id];
+}
+
+// ------------------------------------------------------ staging plan --------
+//
+// A staged tile owns R consecutive rows (R = 16 or 32, one plan each).  The
+// plan lists each tile's distinct columns (sorted, `ucols` from `uoff[tile]`)
+// and rewrites every non-zero's column as its 16-bit position in that list
+// (`lcols`).  The SpMM then copies those X rows into LDS once per tile and
+// reads them from there: R rows along x of a 27-point stencil need 9 (R + 2)
+// distinct X rows, about 10 per row instead of 27 gathers.
+
+constexpr int kStageCap = 1024;  // non-zeros per tile the plan kernel sorts in LDS
+constexpr int kStageLdsBytes = 60 * 1024;  // staged X rows; + the dot array stays <= 64 KiB
+
+static int plan_slot(int rows) { return rows == 16 ? 0 : 1; }
+
+struct SpmmPlan {
+  const mfe_csr* A = nullptr;
+  uint64_t gen = 0;
+  bool built = false, ok = false;
+  int rows = 0, ntiles = 0, maxu = 0;
+  int* uoff = nullptr;
+  int* ucols = nullptr;
+  unsigned short* lcols = nullptr;
+  void release() {
+    (void)hipFree(uoff);
+    (void)hipFree(ucols);
+    (void)hipFree(lcols);
+    uoff = ucols = nullptr;
+    lcols = nullptr;
+    built = ok = false;
+    rows = ntiles = maxu = 0;
+  }
+};
+
+// One workgroup per tile: sort the tile's columns in LDS (bitonic), mark the
+// first of each value; pass 0 counts them, pass 1 writes them and the local
+// indices.  Tiles with more than kStageCap non-zeros report -1.
+__global__ __launch_bounds__(kBlock) void stage_plan_kernel(int n, int rows,
+                                                            const int* __restrict__ ro,
+                                                            const int* __restrict__ cols, int pass,
+                                                            const int* __restrict__ uoff,
+                                                            int* __restrict__ counts,
+                                                            int* __restrict__ ucols,
+                                                            unsigned short* __restrict__ lcols) {
+  __shared__ int a[kStageCap];
+  __shared__ int u[kStageCap];
+  __shared__ int scan[kBlock];
+  const int t = (int)threadIdx.x;
+  const int tile = (int)blockIdx.x;
+  const int r0 = tile * rows, r1 = min(r0 + rows, n);
+  const int s0 = ro[r0], cnt = ro[r1] - s0;
+  if (cnt > kStageCap) {
+    if (pass == 0 && t == 0) counts[tile] = -1;
+    return;
+  }
+  for (int i = t; i < kStageCap; i += kBlock) a[i] = i < cnt ? cols[s0 + i] : INT_MAX;
+  __syncthreads();
+  for (int k = 2; k <= kStageCap; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = t; i < kStageCap; i += kBlock) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const int ai = a[i], aj = a[ixj];
+          if ((i & k) == 0 ? ai > aj : ai < aj) {
+            a[i] = aj;
+            a[ixj] = ai;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  constexpr int kPer = kStageCap / kBlock;  // 4 consecutive entries per thread
+  int f[kPer], lsum = 0;
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int i = kPer * t + q;
+    f[q] = a[i] != INT_MAX && (i == 0 || a[i] != a[i - 1]);
+    lsum += f[q];
+  }
+  scan[t] = lsum;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {  // inclusive scan
+    const int v = t >= off ? scan[t - off] : 0;
+    __syncthreads();
+    scan[t] += v;
+    __syncthreads();
+  }
+  const int total = scan[kBlock - 1];
+  if (pass == 0) {
+    if (t == 0) counts[tile] = total;
+    return;
+  }
+  int r = scan[t] - lsum;
+  const int base = uoff[tile];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q)
+    if (f[q]) {
+      u[r] = a[kPer * t + q];
+      ucols[base + r] = a[kPer * t + q];
+      ++r;
+    }
+  __syncthreads();
+  for (int i = t; i < cnt; i += kBlock) {
+    const int c = cols[s0 + i];
+    int lo = 0, hi = total;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (u[mid] < c) lo = mid + 1;
+      else hi = mid;
+    }
+    lcols[s0 + i] = (unsigned short)lo;
+  }
 }
 
 // Sum over the lanes of one wave that hold the same column (lane % L), in a
@@ -126,18 +246,205 @@ __global__ __launch_bounds__(kBlock) void spmm_kernel(int n, const int* __restri
   }
 }
 
+// Staged SpMM: one workgroup per R-row tile, L = K / CPL lanes per row.  The tile's distinct X rows are copied into LDS (sx[u * K + k], in the
+// plan's column order), then each lane walks its row in chunks of U entries
+// as spmm_kernel does, with the local column index in place of the global
+// one: the same products added in the same order, so the same bits.
+template <int K, int CPL, int U, int R>
+__global__ __launch_bounds__(R * K / CPL) void spmm_staged_kernel(
+    int n, const int* __restrict__ ro, const unsigned short* __restrict__ lcols,
+    const double* __restrict__ vals, const int* __restrict__ uoff, const int* __restrict__ ucols,
+    const double* __restrict__ X, double* __restrict__ Y, double* __restrict__ partials,
+    const int* __restrict__ done) {
+  if (done && *done) return;
+  constexpr int L = K / CPL;
+  constexpr int NT = R * L;
+  constexpr int H = K / 2;  // 16-byte pieces per X row
+  extern __shared__ __align__(16) double sx[];
+  const int t = (int)threadIdx.x;
+  const int tile = (int)blockIdx.x;
+  const int u0 = uoff[tile], nu = uoff[tile + 1] - u0;
+  for (int i = t; i < nu * H; i += NT) {
+    const int u = i / H, h = i - u * H;
+    const double2 d = *reinterpret_cast<const double2*>(X + (int64_t)ucols[u0 + u] * K + 2 * h);
+    *reinterpret_cast<double2*>(sx + u * K + 2 * h) = d;
+  }
+  __syncthreads();
+  const int row = tile * R + t / L;
+  const int k0 = (t % L) * CPL;
+  double acc[CPL];
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) acc[q] = 0.0;
+  if (row < n) {
+    const int a = ro[row], len = ro[row + 1] - a;
+    for (int j0 = 0; j0 < len; j0 += U) {
+      int c[U];
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = a + min(j0 + u, len - 1);
+        c[u] = lcols[j];
+        v[u] = vals[j];
+      }
+      double xv[U][CPL];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const double* src = sx + c[u] * K + k0;
+        if constexpr (CPL % 2 == 0) {
+#pragma unroll
+          for (int q = 0; q < CPL; q += 2) {
+            const double2 d = *reinterpret_cast<const double2*>(src + q);
+            xv[u][q] = d.x;
+            xv[u][q + 1] = d.y;
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < CPL; ++q) xv[u][q] = src[q];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (j0 + u < len) {
+#pragma unroll
+          for (int q = 0; q < CPL; ++q) acc[q] += v[u] * xv[u][q];
+        }
+    }
+    double* dst = Y + (int64_t)row * K + k0;
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) dst[q] = acc[q];
+  }
+  if (partials) {
+    constexpr int NW = NT / kWave;
+    __shared__ double red[NW][K];
+    double term[CPL];
+#pragma unroll
+    for (int q = 0; q < CPL; ++q)
+      term[q] = row < n ? acc[q] * X[(int64_t)row * K + k0 + q] : 0.0;
+    const int lane = t % kWave, wave = t / kWave;
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      const double s = sum_same_column<L>(term[q]);
+      if (lane < L) red[wave][lane * CPL + q] = s;
+    }
+    __syncthreads();
+    if (t < K) {
+      double s = red[0][t];
+#pragma unroll
+      for (int w = 1; w < NW; ++w) s += red[w][t];
+      partials[(int64_t)tile * K + t] = s;
+    }
+  }
+}
+
+// Builds the R-row staging plan of A (synchronises ctx's stream).  A matrix
+// with a tile of more than kStageCap non-zeros gets ok = false (its staged
+// tiles of that R are then not offered); so does n = 0.
+static int build_stage_plan(mfe_ctx* ctx, const mfe_csr* A, int R, SpmmPlan& p) {
+  p.release();
+  p.A = A;
+  p.gen = A->plan_gen;
+  p.built = true;
+  p.rows = R;
+  const int64_t n = A->nrows;
+  if (n == 0 || n > INT_MAX - R) return MFE_OK;
+  const int nt = (int)cdiv(n, R);
+  hipStream_t s = ctx->stream;
+  MFE_CHECK(ctx, hipMalloc(&p.uoff, sizeof(int) * ((size_t)nt + 1)));
+  stage_plan_kernel<<<nt, kBlock, 0, s>>>((int)n, R, A->row_offsets, A->cols, 0, nullptr, p.uoff,
+                                          nullptr, nullptr);
+  MFE_LAUNCHED(ctx, "stage_plan_kernel count");
+  std::vector<int> h(nt + 1);
+  MFE_CHECK(ctx, hipMemcpyAsync(h.data(), p.uoff, sizeof(int) * nt, hipMemcpyDeviceToHost, s));
+  MFE_CHECK(ctx, hipStreamSynchronize(s));
+  int64_t total = 0;
+  int maxu = 0;
+  for (int i = 0; i < nt; ++i) {
+    if (h[i] < 0) {  // a tile too long for the plan kernel
+      p.release();
+      p.A = A;
+      p.gen = A->plan_gen;
+      p.built = true;
+      p.rows = R;
+      return MFE_OK;
+    }
+    const int c = h[i];
+    h[i] = (int)total;
+    total += c;
+    maxu = std::max(maxu, c);
+  }
+  h[nt] = (int)total;
+  if (total > INT_MAX) return MFE_OK;
+  MFE_CHECK(ctx, hipMemcpyAsync(p.uoff, h.data(), sizeof(int) * ((size_t)nt + 1),
+                                hipMemcpyHostToDevice, s));
+  MFE_CHECK(ctx, hipMalloc(&p.ucols, sizeof(int) * (size_t)std::max<int64_t>(total, 1)));
+  MFE_CHECK(ctx, hipMalloc(&p.lcols, sizeof(unsigned short) * (size_t)std::max<int64_t>(A->nnz, 1)));
+  stage_plan_kernel<<<nt, kBlock, 0, s>>>((int)n, R, A->row_offsets, A->cols, 1, p.uoff, nullptr,
+                                          p.ucols, p.lcols);
+  MFE_LAUNCHED(ctx, "stage_plan_kernel fill");
+  MFE_CHECK(ctx, hipStreamSynchronize(s));
+  p.ntiles = nt;
+  p.maxu = maxu;
+  p.ok = true;
+  return MFE_OK;
+}
+
+// Operands of one SpMM launch; plans[plan_slot(R)] is the R-row staging plan
+// (null: staged tiles of that R unavailable).
+struct SpmmArgs {
+  int n;
+  const int* ro;
+  const int* cols;
+  const double* vals;
+  const SpmmPlan* plans[2];
+};
+
+static bool stage_ok(const SpmmPlan* p, int K) {
+  return p && p->ok && (int64_t)p->maxu * K * (int64_t)sizeof(double) <= kStageLdsBytes;
+}
+
+// Staged shapes need whole waves and at most 1024 threads: 64 <= R (K / CPL).
+template <int K, int CPL, int U, int R>
+static hipError_t launch_staged(const SpmmArgs& a, size_t lds, const double* X, double* Y,
+                                double* partials, const int* done, hipStream_t s) {
+  constexpr int NT = CPL <= K ? R * (K / CPL) : 0;
+  if constexpr (NT >= kWave && NT <= 1024) {
+    const SpmmPlan& p = *a.plans[plan_slot(R)];
+    spmm_staged_kernel<K, CPL, U, R><<<p.ntiles, NT, lds, s>>>(a.n, a.ro, p.lcols, a.vals, p.uoff,
+                                                                 p.ucols, X, Y, partials, done);
+    return hipGetLastError();
+  } else {
+    return hipErrorInvalidValue;
+  }
+}
+
 template <int K>
-static hipError_t launch_spmm(int tile, int n, const int* ro, const int* cols, const double* vals,
-                              const double* X, double* Y, double* partials, const int* done,
-                              hipStream_t s, int* blocks_out) {
+static hipError_t launch_spmm(int tile, const SpmmArgs& a, const double* X, double* Y,
+                              double* partials, const int* done, hipStream_t s, int* blocks_out) {
   const SpmmTile T = tile_of(K, tile);
+  const int n = a.n;
+  if (T.rows) {
+    const SpmmPlan* p = a.plans[plan_slot(T.rows)];
+    if (!stage_ok(p, K) || p->rows != T.rows) return hipErrorInvalidValue;
+    if (blocks_out) *blocks_out = p->ntiles;
+    const size_t lds = (size_t)p->maxu * K * sizeof(double);
+#define MFE_STAGED_CASE(C, UU, RR)                  \
+    if (T.cpl == C && T.unroll == UU && T.rows == RR) \
+      return launch_staged<K, C, UU, RR>(a, lds, X, Y, partials, done, s);
+    MFE_STAGED_CASE(1, 9, 16)
+    MFE_STAGED_CASE(2, 9, 16)
+    MFE_STAGED_CASE(1, 9, 32)
+    MFE_STAGED_CASE(2, 9, 32)
+    MFE_STAGED_CASE(4, 9, 32)
+#undef MFE_STAGED_CASE
+    return hipErrorInvalidValue;
+  }
   const int rpb = kBlock / (K / T.cpl);
   const unsigned g = (unsigned)cdiv(n, rpb);
   if (blocks_out) *blocks_out = (int)g;
   if (n == 0) return hipSuccess;
 #define MFE_SPMM_CASE(C, UU)                                                                \
   if (T.cpl == C && T.unroll == UU) {                                                      \
-    spmm_kernel<K, (C <= K ? C : K), UU><<<g, kBlock, 0, s>>>(n, ro, cols, vals, X, Y,      \
+    spmm_kernel<K, (C <= K ? C : K), UU><<<g, kBlock, 0, s>>>(n, a.ro, a.cols, a.vals, X, Y, \
                                                                partials, done);            \
     return hipGetLastError();                                                              \
   }
@@ -153,18 +460,24 @@ static hipError_t launch_spmm(int tile, int n, const int* ro, const int* cols, c
   return hipErrorInvalidValue;
 }
 
-static hipError_t launch_spmm_k(int K, int tile, int n, const int* ro, const int* cols,
-                                const double* vals, const double* X, double* Y, double* partials,
-                                const int* done, hipStream_t s, int* blocks_out) {
+static hipError_t launch_spmm_k(int K, int tile, const SpmmArgs& a, const double* X, double* Y,
+                                double* partials, const int* done, hipStream_t s,
+                                int* blocks_out) {
   switch (K) {
-    case 16: return launch_spmm<16>(tile, n, ro, cols, vals, X, Y, partials, done, s, blocks_out);
-    case 8: return launch_spmm<8>(tile, n, ro, cols, vals, X, Y, partials, done, s, blocks_out);
-    case 4: return launch_spmm<4>(tile, n, ro, cols, vals, X, Y, partials, done, s, blocks_out);
+    case 16: return launch_spmm<16>(tile, a, X, Y, partials, done, s, blocks_out);
+    case 8: return launch_spmm<8>(tile, a, X, Y, partials, done, s, blocks_out);
+    case 4: return launch_spmm<4>(tile, a, X, Y, partials, done, s, blocks_out);
     default: return hipErrorInvalidValue;
   }
 }
 
 static int k_slot(int K) { return K == 16 ? 2 : K == 8 ? 1 : 0; }
+
+// Tile `tile` can run on A for K with these plans.
+static bool tile_available(int K, int tile, const SpmmArgs& a) {
+  const int R = tile_of(K, tile).rows;
+  return R == 0 || stage_ok(a.plans[plan_slot(R)], K);
+}
 
 // ---------------------------------------------------------- autotuner -------
 
@@ -172,26 +485,29 @@ struct SpmmTuning {
   const mfe_csr* A = nullptr;
   uint64_t gen = 0;
   int tile[3] = {-1, -1, -1};
-  double ms[3][16] = {};
+  double ms[3][kMaxTiles] = {};
 };
 
-// Times every tile for K on A (3 launches each after one warm-up, HIP events
-// on ctx's stream) and keeps the fastest.  X and Y are scratch of n*K.
-static int tune_spmm(mfe_ctx* ctx, const mfe_csr* A, int K, const double* X, double* Y,
+// Times every available tile for K on A (3 launches each after one warm-up,
+// HIP events on ctx's stream) and keeps the fastest; unavailable tiles get
+// ms = +inf.  X and Y are scratch of n*K.
+static int tune_spmm(mfe_ctx* ctx, const SpmmArgs& a, int K, const double* X, double* Y,
                      SpmmTuning& tun) {
   hipStream_t s = ctx->stream;
   const int nt = tile_count(K);
   const int slot = k_slot(K);
   int best = 0;
-  double best_ms = 1e300;
+  double best_ms = HUGE_VAL;
   for (int id = 0; id < nt; ++id) {
-    MFE_CHECK(ctx, launch_spmm_k(K, id, (int)A->nrows, A->row_offsets, A->cols, A->vals, X, Y,
-                                 nullptr, nullptr, s, nullptr));
+    if (!tile_available(K, id, a)) {
+      tun.ms[slot][id] = HUGE_VAL;
+      continue;
+    }
+    MFE_CHECK(ctx, launch_spmm_k(K, id, a, X, Y, nullptr, nullptr, s, nullptr));
     MFE_CHECK(ctx, hipEventRecord(ctx->t_start, s));
     const int reps = 3;
     for (int r = 0; r < reps; ++r)
-      MFE_CHECK(ctx, launch_spmm_k(K, id, (int)A->nrows, A->row_offsets, A->cols, A->vals, X, Y,
-                                   nullptr, nullptr, s, nullptr));
+      MFE_CHECK(ctx, launch_spmm_k(K, id, a, X, Y, nullptr, nullptr, s, nullptr));
     MFE_CHECK(ctx, hipEventRecord(ctx->t_stop, s));
     MFE_CHECK(ctx, hipEventSynchronize(ctx->t_stop));
     float ms = 0;
@@ -403,6 +719,7 @@ struct MultiWorkspace {
   uint64_t kgen = 0;
   int ktile = -1;
   SpmmTuning tun;
+  SpmmPlan plans[2];  // 16- and 32-row staging plans of one matrix (kept across nk changes)
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -430,6 +747,34 @@ struct MultiWorkspace {
 static unsigned multi_vec_grid(int64_t nk) {
   const int64_t b = cdiv(nk, kBlock);
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, kVecBlocks));
+}
+
+// The R-row staging plan of A in ctx's workspace, rebuilt when A or its
+// structure changed.  A rebuilt plan moves its arrays, so the captured CG
+// graph (which holds their addresses) is dropped with it.
+static int ensure_plan(mfe_ctx* ctx, const mfe_csr* A, int R) {
+  if (!ctx->multi) ctx->multi = new MultiWorkspace();
+  MultiWorkspace* w = ctx->multi;
+  SpmmPlan& p = w->plans[plan_slot(R)];
+  if (!p.built || p.A != A || p.gen != A->plan_gen) {
+    w->drop_graph();
+    const int st = build_stage_plan(ctx, A, R, p);
+    if (st) {
+      p.release();
+      return st;
+    }
+  }
+  return MFE_OK;
+}
+
+// Operands for A with the workspace's plans that belong to A (others null).
+static SpmmArgs spmm_args(const mfe_csr* A, const MultiWorkspace* w) {
+  SpmmArgs a{(int)A->nrows, A->row_offsets, A->cols, A->vals, {nullptr, nullptr}};
+  if (w)
+    for (int i = 0; i < 2; ++i)
+      if (w->plans[i].built && w->plans[i].A == A && w->plans[i].gen == A->plan_gen)
+        a.plans[i] = &w->plans[i];
+  return a;
 }
 
 static int multi_prepare(mfe_ctx* ctx, int64_t n, int K, MultiWorkspace** out) {
@@ -485,8 +830,7 @@ static hipError_t enqueue_multi_iteration(MultiWorkspace* w, const mfe_csr* A, d
   const int* done = &w->st->done;
   multi_p_kernel<K><<<vg, kBlock, 0, s>>>(nk, w->st, w->R, w->P);
   int nb = 0;
-  hipError_t e = launch_spmm<K>(tile, (int)A->nrows, A->row_offsets, A->cols, A->vals, w->P,
-                                w->AP, w->part, done, s, &nb);
+  hipError_t e = launch_spmm<K>(tile, spmm_args(A, w), w->P, w->AP, w->part, done, s, &nb);
   if (e != hipSuccess) return e;
   if ((e = reduce_cols(K, w->part, nb, w->part2, w->res, done, s)) != hipSuccess) return e;
   multi_alpha_kernel<<<1, 64, 0, s>>>(w->st, K, w->res);
@@ -521,6 +865,8 @@ static bool k_ok(int K) { return K == 4 || K == 8 || K == 16; }
 
 static int ensure_tuned(mfe_ctx* ctx, const mfe_csr* A, int K, MultiWorkspace* w) {
   const int slot = k_slot(K);
+  int st;
+  if ((st = ensure_plan(ctx, A, 16)) || (st = ensure_plan(ctx, A, 32))) return st;
   if (w->tun.A == A && w->tun.gen == A->plan_gen && w->tun.tile[slot] >= 0) return MFE_OK;
   if (w->tun.A != A || w->tun.gen != A->plan_gen) {
     w->tun = SpmmTuning();
@@ -529,7 +875,7 @@ static int ensure_tuned(mfe_ctx* ctx, const mfe_csr* A, int K, MultiWorkspace* w
   }
   // scratch operands: R (filled with ones) and AP
   MFE_CHECK(ctx, hipMemsetAsync(w->R, 0, sizeof(double) * (size_t)w->nk, ctx->stream));
-  return tune_spmm(ctx, A, K, w->R, w->AP, w->tun);
+  return tune_spmm(ctx, spmm_args(A, w), K, w->R, w->AP, w->tun);
 }
 
 }  // namespace mfe
@@ -554,12 +900,33 @@ int mfe_spmm_tune(mfe_ctx* ctx, const mfe_csr* A, int nrhs, int* tile, double* m
   return MFE_OK;
 }
 
-int mfe_spmm_tile_info(int nrhs, int tile, int* cols_per_lane, int* unroll, int* rows_per_block) {
+int mfe_spmm_select(mfe_ctx* ctx, const mfe_csr* A, int nrhs, int tile) {
+  if (!ctx || !A || !k_ok(nrhs)) return MFE_BAD_ARG;
+  if (tile < 0 || tile >= tile_count(nrhs)) return fail(ctx, MFE_BAD_ARG, "spmm_select: no such tile");
+  MFE_CHECK(ctx, hipSetDevice(ctx->device));
+  MultiWorkspace* w = nullptr;
+  int st;
+  if ((st = multi_prepare(ctx, A->nrows, nrhs, &w))) return st;
+  if ((st = ensure_plan(ctx, A, 16)) || (st = ensure_plan(ctx, A, 32))) return st;
+  if (!tile_available(nrhs, tile, spmm_args(A, w)))
+    return fail(ctx, MFE_BAD_ARG, "spmm_select: staged tile unavailable for this matrix");
+  if (w->tun.A != A || w->tun.gen != A->plan_gen) {
+    w->tun = SpmmTuning();
+    w->tun.A = A;
+    w->tun.gen = A->plan_gen;
+  }
+  w->tun.tile[k_slot(nrhs)] = tile;
+  return MFE_OK;
+}
+
+int mfe_spmm_tile_info(int nrhs, int tile, int* cols_per_lane, int* unroll, int* rows_per_block,
+                       int* staged) {
   if (!k_ok(nrhs) || tile < 0 || tile >= tile_count(nrhs)) return MFE_BAD_ARG;
   const SpmmTile T = tile_of(nrhs, tile);
   if (cols_per_lane) *cols_per_lane = T.cpl;
   if (unroll) *unroll = T.unroll;
-  if (rows_per_block) *rows_per_block = kBlock / (nrhs / T.cpl);
+  if (rows_per_block) *rows_per_block = T.rows ? T.rows : kBlock / (nrhs / T.cpl);
+  if (staged) *staged = T.rows ? 1 : 0;
   return MFE_OK;
 }
 
@@ -576,8 +943,16 @@ int mfe_spmm(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* X, mfe_vec* Y, int n
       return st;
     tile = w->tun.tile[k_slot(nrhs)];
   }
-  MFE_CHECK(ctx, launch_spmm_k(nrhs, tile, (int)A->nrows, A->row_offsets, A->cols, A->vals, X->d,
-                               Y->d, nullptr, nullptr, ctx->stream, nullptr));
+  const int R = tile_of(nrhs, tile).rows;
+  if (R) {
+    if ((st = ensure_plan(ctx, A, R))) return st;
+    if (!stage_ok(&ctx->multi->plans[plan_slot(R)], nrhs))
+      return fail(ctx, MFE_BAD_ARG,
+                  "spmm: staged tile unavailable for this matrix (a row tile has more than "
+                  "1024 non-zeros or its distinct X rows exceed 64 KiB)");
+  }
+  MFE_CHECK(ctx, launch_spmm_k(nrhs, tile, spmm_args(A, ctx->multi), X->d, Y->d, nullptr, nullptr,
+                               ctx->stream, nullptr));
   return MFE_OK;
 }
 
@@ -602,8 +977,8 @@ int mfe_cg_solve_multi(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* B, mfe_vec
   if (n > 0) {
     // R = B - A X ; tolerances from ||B[:,k]||, rtrans = ||R[:,k]||^2
     int nb = 0;
-    MFE_CHECK(ctx, launch_spmm_k(K, tile, (int)n, A->row_offsets, A->cols, A->vals, X->d, w->AP,
-                                 nullptr, nullptr, s, &nb));
+    MFE_CHECK(ctx, launch_spmm_k(K, tile, spmm_args(A, w), X->d, w->AP, nullptr, nullptr, s,
+                                 &nb));
     multi_resid_kernel<<<multi_vec_grid(nk), kBlock, 0, s>>>(nk, B->d, w->AP, w->R);
     MFE_CHECK(ctx, launch_multi_dot(K, nk, B->d, B->d, w->part, s));
     MFE_CHECK(ctx, reduce_cols(K, w->part, (int)multi_vec_grid(nk), w->part2, w->res + 32, nullptr, s));
@@ -660,6 +1035,8 @@ int mfe_cg_solve_multi(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* B, mfe_vec
 void mfe_multi_workspace_free(mfe_ctx* ctx) {
   if (ctx && ctx->multi) {
     ctx->multi->release();
+    ctx->multi->plans[0].release();
+    ctx->multi->plans[1].release();
     delete ctx->multi;
     ctx->multi = nullptr;
   }

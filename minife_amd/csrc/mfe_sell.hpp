@@ -87,4 +87,69 @@ __device__ __forceinline__ T sell_row_sum(const int* cols, const T* vals, const 
                  : sell_row_sum_w<T, false>(cols, vals, x, o64, w, lane, len, sum);
 }
 
+typedef unsigned short sell_us2 __attribute__((ext_vector_type(2)));
+
+// A lane's slots with 16-bit columns, loaded by the same instructions for
+// either parity of the width: s = the single slot 0 of an odd width (for an
+// even width a load of pair 0's address, unused), p[k] = pair k (index
+// clamped into the slice).  Kept raw so that no wait for the loads is needed
+// before the values are used (sell_slot picks slot j afterwards).
+template <typename T>
+struct SellLocalRaw {
+  int odd;
+  unsigned short c0;
+  T v0;
+  sell_us2 cp[kSellMaxWidth / 2];
+  typename SellVec2<T>::type vp[kSellMaxWidth / 2];
+};
+
+template <typename T>
+__device__ __forceinline__ void sell_load_local(const unsigned short* lcols, const T* vals,
+                                                int64_t o64, int w, int lane, SellLocalRaw<T>& r) {
+  using V2 = typename SellVec2<T>::type;
+  constexpr int kPairs = kSellMaxWidth / 2;  // 13: slots 1..26 (odd) or 0..25 (even)
+  r.odd = w & 1;
+  const int np = (w - r.odd) >> 1;
+  const int last = np > 0 ? np - 1 : 0;
+  const int64_t pb = o64 + (r.odd ? 64 : 0);
+  r.c0 = lcols[o64 + lane];
+  r.v0 = vals[o64 + lane];
+#pragma unroll
+  for (int p = 0; p < kPairs; ++p) {
+    const int64_t at = pb + (int64_t)(p < last ? p : last) * 128 + lane * 2;
+    r.cp[p] = *reinterpret_cast<const sell_us2*>(lcols + at);
+    r.vp[p] = *reinterpret_cast<const V2*>(vals + at);
+  }
+}
+
+// Slot j (a compile-time index after unrolling) of a raw load: column and
+// value.  Odd widths: slot 0 single, slots 2k+1, 2k+2 = pair k; even widths:
+// slots 2k, 2k+1 = pair k (slot 26 does not exist: column 0, value 0).
+template <typename T>
+__device__ __forceinline__ void sell_slot(const SellLocalRaw<T>& r, int j, int& c, T& v) {
+  constexpr int kPairs = kSellMaxWidth / 2;
+  int ce, co;
+  T ve, vo;
+  if (j == 0) {
+    co = r.c0;
+    vo = r.v0;
+    ce = r.cp[0].x;
+    ve = r.vp[0].x;
+  } else if (j & 1) {
+    const int k = (j - 1) >> 1;
+    co = r.cp[k].x;
+    vo = r.vp[k].x;
+    ce = r.cp[k].y;
+    ve = r.vp[k].y;
+  } else {
+    const int k = (j - 2) >> 1;
+    co = r.cp[k].y;
+    vo = r.vp[k].y;
+    ce = (j >> 1) < kPairs ? (int)r.cp[j >> 1].x : 0;
+    ve = (j >> 1) < kPairs ? r.vp[j >> 1].x : T(0);
+  }
+  c = r.odd ? co : ce;
+  v = r.odd ? vo : ve;
+}
+
 }  // namespace mfe

@@ -15,6 +15,7 @@
 // of the mixed-precision refinement.
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <type_traits>
@@ -211,8 +212,253 @@ __global__ __launch_bounds__(kBlock) void sell_fill_kernel(
   }
 }
 
+// ---------------------------------------------- SELL-64 with staged x -------
+//
+// Workgroup = 4 slices = a tile of 256 rows.  A plan lists each tile's
+// distinct columns in increasing order, as runs of consecutive columns (first
+// column, position in the list), and stores at every SELL slot the column's
+// position in the tile's list as 16 bits.  The SpMV first issues the lane's
+// value and local-column loads (the HBM stream), then copies the tile's runs
+// of x into LDS (coalesced: a 256-row tile of the 27-point stencil has 9 runs,
+// 2.3 K entries), and after the barrier gathers x from LDS.  Index traffic
+// drops from 4 to 2 bytes per slot and x comes from L2 once per tile instead
+// of once per slot.  Sums are in stored column order: the other kernels' bits.
+
+constexpr int kXCap = 8192;        // non-zeros per tile the plan kernel sorts in LDS
+constexpr int kXMaxRuns = 128;     // runs per tile the SpMV keeps in LDS
+constexpr int kXLdsBytes = 56 * 1024;  // staged x per tile (27 * 256 doubles fit)
+constexpr int kXFirst = 10;  // x entries per thread requested before the slot loads
+                             // (2560 per tile: a 27-point tile needs 9 * 258)
+
+// Exclusive workgroup scan of two counts; totals in ta, tb.
+__device__ __forceinline__ int2 block_scan2(int a, int b, int* sa, int* sb, int& ta, int& tb) {
+  const int t = (int)threadIdx.x;
+  sa[t] = a;
+  sb[t] = b;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    const int va = t >= off ? sa[t - off] : 0, vb = t >= off ? sb[t - off] : 0;
+    __syncthreads();
+    sa[t] += va;
+    sb[t] += vb;
+    __syncthreads();
+  }
+  ta = sa[kBlock - 1];
+  tb = sb[kBlock - 1];
+  return make_int2(sa[t] - a, sb[t] - b);
+}
+
+// One workgroup per tile: the tile's columns sorted in LDS (bitonic); a value
+// is distinct where it differs from its predecessor and starts a run where the
+// predecessor is not value - 1.  Pass 0: counts[tile] = {distinct, runs}, or
+// {-1, -1} past kXCap non-zeros.  Pass 1: the runs at roff[tile], and the
+// local columns at every slot of the tile's slices (padding slots: 0).
+__global__ __launch_bounds__(kBlock) void sellx_plan_kernel(
+    int n, int nslices, const int* __restrict__ ro, const int* __restrict__ cols,
+    const int* __restrict__ soff, const unsigned char* __restrict__ slen, int pass,
+    int2* __restrict__ counts, const int* __restrict__ roff, int* __restrict__ rcol,
+    int* __restrict__ rpos, unsigned short* __restrict__ lcols) {
+  __shared__ int a[kXCap];
+  __shared__ int sa[kBlock], sb[kBlock];
+  constexpr int kPer = kXCap / kBlock;
+  const int t = (int)threadIdx.x, tile = (int)blockIdx.x;
+  const int r0 = tile * kSpmvRows, r1 = min(r0 + kSpmvRows, n);
+  const int s0 = ro[r0], cnt = ro[r1] - s0;
+  if (cnt > kXCap) {
+    if (pass == 0 && t == 0) counts[tile] = make_int2(-1, -1);
+    return;
+  }
+  for (int i = t; i < kXCap; i += kBlock) a[i] = i < cnt ? cols[s0 + i] : INT_MAX;
+  __syncthreads();
+  for (int k = 2; k <= kXCap; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = t; i < kXCap; i += kBlock) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const int ai = a[i], aj = a[ixj];
+          if ((i & k) == 0 ? ai > aj : ai < aj) {
+            a[i] = aj;
+            a[ixj] = ai;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  int v[kPer];
+  unsigned distinct = 0, starts = 0;
+  int nu = 0, nr = 0;
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int i = kPer * t + q;
+    v[q] = a[i];
+    const int prev = i > 0 ? a[i - 1] : INT_MIN;
+    const bool d = v[q] != INT_MAX && v[q] != prev;
+    const bool r = d && (i == 0 || prev != v[q] - 1);
+    distinct |= (unsigned)d << q;
+    starts |= (unsigned)r << q;
+    nu += d;
+    nr += r;
+  }
+  int tu, tr;
+  const int2 base = block_scan2(nu, nr, sa, sb, tu, tr);
+  if (pass == 0) {
+    if (t == 0) counts[tile] = make_int2(tu, tr);
+    return;
+  }
+  __syncthreads();  // every thread holds its entries: compact the list in place
+  int u = base.x, r = base.y;
+  const int rb = roff[tile];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q)
+    if ((distinct >> q) & 1) {
+      if ((starts >> q) & 1) {
+        rcol[rb + r] = v[q];
+        rpos[rb + r] = u;
+        ++r;
+      }
+      a[u] = v[q];
+      ++u;
+    }
+  __syncthreads();
+  const int slice = tile * kSlicesPerBlock + t / kSlice;
+  if (slice >= nslices) return;
+  const int row = tile * kSpmvRows + t;
+  const int o = soff[slice], w = soff[slice + 1] - o;
+  const int len = slen[row];
+  const int start = row < n ? ro[row] : 0;
+  for (int j = 0; j < w; ++j) {
+    int loc = 0;
+    if (j < len) {
+      const int c = cols[start + j];
+      int lo = 0, hi = tu;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < c) lo = mid + 1;
+        else hi = mid;
+      }
+      loc = lo;
+    }
+    lcols[sell_pos((int64_t)o * kSlice, j, w, t % kSlice)] = (unsigned short)loc;
+  }
+}
+
+// spmv_sell_kernel's modes and results, x staged per tile (dynamic LDS:
+// A->sellx_maxu entries of T).  Order of work: the tile's run list goes to
+// LDS; the thread's first kXFirst x entries are requested, then the lane's
+// slot loads (memory returns loads in issue order, so the x entries can be
+// written to LDS while the slot loads are still in flight); remaining x
+// entries, if any, follow; after the barrier the slots are unpacked and
+// summed.  Every local column in the array is below A->sellx_maxu, so the LDS
+// reads of padding and of clamped loads stay inside the allocation.
+template <typename T, bool ZERO_Y, int MODE>
+__global__ __launch_bounds__(kBlock) void spmv_sellx_kernel(
+    int n, int nslices, const int* __restrict__ soff, const unsigned char* __restrict__ slen,
+    const unsigned short* __restrict__ lcols, const T* __restrict__ vals,
+    const int* __restrict__ roff, const int* __restrict__ rcol, const int* __restrict__ rpos,
+    const int* __restrict__ tnu, const T* __restrict__ x, T* __restrict__ y,
+    const T* __restrict__ bvec, T* __restrict__ partials, const int* __restrict__ done) {
+  if (done && *done) return;
+  constexpr int CH = sizeof(T) == 8 ? 8 : 4;  // x loads in flight per thread while staging
+  extern __shared__ __align__(16) unsigned char sx_bytes[];
+  T* sx = reinterpret_cast<T*>(sx_bytes);
+  __shared__ int run_col[kXMaxRuns], run_pos[kXMaxRuns + 1];
+  __shared__ T red[4];
+  const int t = (int)threadIdx.x, tile = (int)blockIdx.x;
+  const int ra = roff[tile], nr = roff[tile + 1] - ra, nu = tnu[tile];
+  const bool has_run = t < nr;
+  const int my_col = has_run ? rcol[ra + t] : 0;
+  const int my_pos = has_run ? rpos[ra + t] : 0;
+  const int slice = tile * kSlicesPerBlock + t / kSlice;
+  const int lane = t % kSlice;
+  const int row = slice * kSlice + lane;
+  const bool live = slice < nslices;  // uniform per wave
+  const bool valid = live && row < n;
+  const int rr = valid ? row : n - 1;
+  const int sl = live ? slice : nslices - 1;
+  const T y0 = ZERO_Y ? y[rr] : T(0);
+  const T xr = MODE == kDot ? x[rr] : T(0);
+  const T br = MODE == kResid ? bvec[rr] : T(0);
+  const int o = soff[sl];
+  const int w = live ? soff[sl + 1] - o : 0;
+  const int len = live ? (int)slen[row] : 0;
+  if (has_run) {
+    run_col[t] = my_col;
+    run_pos[t] = my_pos;
+  }
+  if (t == 0) run_pos[nr] = nu;
+  __syncthreads();
+  int r = 0;  // run of entry i; i grows, so the run only moves forward
+  T xf[kXFirst];
+  {
+    int g[kXFirst];
+#pragma unroll
+    for (int k = 0; k < kXFirst; ++k) {
+      const int i = t + k * kBlock;
+      while (r + 1 < nr && run_pos[r + 1] <= i) ++r;
+      g[k] = i < nu ? run_col[r] + (i - run_pos[r]) : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kXFirst; ++k) xf[k] = x[g[k]];
+  }
+  SellLocalRaw<T> raw;
+  sell_load_local<T>(lcols, vals, (int64_t)o * kSlice, w, lane, raw);
+#pragma unroll
+  for (int k = 0; k < kXFirst; ++k)
+    if (t + k * kBlock < nu) sx[t + k * kBlock] = xf[k];
+  for (int i0 = t + kXFirst * kBlock; i0 < nu; i0 += kBlock * CH) {
+    int g[CH];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      const int i = i0 + k * kBlock;
+      while (r + 1 < nr && run_pos[r + 1] <= i) ++r;
+      g[k] = i < nu ? run_col[r] + (i - run_pos[r]) : 0;
+    }
+    T xv[CH];
+#pragma unroll
+    for (int k = 0; k < CH; ++k) xv[k] = x[g[k]];
+#pragma unroll
+    for (int k = 0; k < CH; ++k)
+      if (i0 + k * kBlock < nu) sx[i0 + k * kBlock] = xv[k];
+  }
+  __syncthreads();
+  T sum = T(0);
+  if (live) {
+    if (ZERO_Y) sum = T(0) * y0;
+    int c[kSellMaxWidth];
+    T v[kSellMaxWidth], xv[kSellMaxWidth];
+#pragma unroll
+    for (int j = 0; j < kSellMaxWidth; ++j) sell_slot<T>(raw, j, c[j], v[j]);
+#pragma unroll
+    for (int j = 0; j < kSellMaxWidth; ++j) xv[j] = sx[c[j]];
+#pragma unroll
+    for (int j = 0; j < kSellMaxWidth; ++j)
+      if (j < len) sum += v[j] * xv[j];
+    if (MODE == kResid) sum = T(1) * br + T(-1) * sum;
+    if (valid) y[row] = sum;
+  }
+  if (MODE != kPlain) {
+    const T term = valid ? (MODE == kDot ? sum * xr : sum * sum) : T(0);
+    const T v0 = block_sum<T>(term, red);
+    if (t == 0) partials[blockIdx.x] = v0;
+  }
+}
+
+static void sellx_release(mfe_csr* A) {
+  (void)hipFree(A->sellx_lcols);
+  (void)hipFree(A->sellx_roff);
+  (void)hipFree(A->sellx_rcol);
+  (void)hipFree(A->sellx_rpos);
+  (void)hipFree(A->sellx_nu);
+  A->sellx_lcols = nullptr;
+  A->sellx_roff = A->sellx_rcol = A->sellx_rpos = A->sellx_nu = nullptr;
+  A->sellx_valid = A->sellx_ok = false;
+  A->sellx_maxu = 0;
+  A->sellx_ntiles = A->sellx_runs = 0;
+}
+
 void sell_release(mfe_csr* A) {
-  ++A->plan_gen;
+  A->plan_gen = next_plan_gen();
+  sellx_release(A);
   hipFree(A->sell_off);
   hipFree(A->sell_len);
   hipFree(A->sell_cols);
@@ -248,6 +494,71 @@ static bool try_alloc(P** p, size_t bytes) {
   return false;
 }
 
+// The staged-x plan of A's SELL structure (synchronises the stream).  Leaves
+// sellx_ok false, and the plain SELL kernel in use, when a tile has more than
+// kXMaxRuns runs or kXCap non-zeros, or the arrays do not fit in memory.
+static int build_sellx(mfe_ctx* ctx, mfe_csr* A) {
+  sellx_release(A);
+  A->sellx_valid = true;
+  const int64_t n = A->nrows;
+  if (!A->sell_ok || n == 0) return MFE_OK;
+  hipStream_t s = ctx->stream;
+  const int nt = (int)cdiv(n, kSpmvRows);
+  const int ns = (int)A->sell_nslices;
+  int2* d_counts = nullptr;
+  if (!try_alloc(&d_counts, sizeof(int2) * (size_t)nt)) return MFE_OK;
+  sellx_plan_kernel<<<nt, kBlock, 0, s>>>((int)n, ns, A->row_offsets, A->cols, A->sell_off,
+                                           A->sell_len, 0, d_counts, nullptr, nullptr, nullptr,
+                                           nullptr);
+  std::vector<int2> counts((size_t)nt);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(counts.data(), d_counts, sizeof(int2) * (size_t)nt, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(d_counts);
+  if (e != hipSuccess) return hip_fail(ctx, e, "sellx_plan_kernel count");
+  std::vector<int> roff((size_t)nt + 1), nu((size_t)nt);
+  int64_t runs = 0;
+  int maxu = 0;
+  for (int i = 0; i < nt; ++i) {
+    const int2 c = counts[(size_t)i];
+    if (c.x < 0 || c.y > kXMaxRuns) return MFE_OK;
+    roff[(size_t)i] = (int)runs;
+    runs += c.y;
+    nu[(size_t)i] = c.x;
+    maxu = std::max(maxu, c.x);
+  }
+  roff[(size_t)nt] = (int)runs;
+  if ((int64_t)maxu * (int64_t)sizeof(double) > kXLdsBytes || runs >= INT32_MAX) return MFE_OK;
+  const size_t nl = (size_t)(A->sell_slots + kSellTail);
+  const bool ok = try_alloc(&A->sellx_lcols, sizeof(unsigned short) * nl) &&
+                  try_alloc(&A->sellx_roff, sizeof(int) * ((size_t)nt + 1)) &&
+                  try_alloc(&A->sellx_rcol, sizeof(int) * (size_t)std::max<int64_t>(runs, 1)) &&
+                  try_alloc(&A->sellx_rpos, sizeof(int) * (size_t)std::max<int64_t>(runs, 1)) &&
+                  try_alloc(&A->sellx_nu, sizeof(int) * (size_t)nt);
+  if (!ok) {
+    sellx_release(A);
+    A->sellx_valid = true;
+    return MFE_OK;
+  }
+  MFE_CHECK(ctx, hipMemsetAsync(A->sellx_lcols + A->sell_slots, 0,
+                                sizeof(unsigned short) * kSellTail, s));
+  MFE_CHECK(ctx, hipMemcpyAsync(A->sellx_roff, roff.data(), sizeof(int) * roff.size(),
+                                hipMemcpyHostToDevice, s));
+  MFE_CHECK(ctx, hipMemcpyAsync(A->sellx_nu, nu.data(), sizeof(int) * nu.size(),
+                                hipMemcpyHostToDevice, s));
+  sellx_plan_kernel<<<nt, kBlock, 0, s>>>((int)n, ns, A->row_offsets, A->cols, A->sell_off,
+                                           A->sell_len, 1, nullptr, A->sellx_roff, A->sellx_rcol,
+                                           A->sellx_rpos, A->sellx_lcols);
+  MFE_LAUNCHED(ctx, "sellx_plan_kernel fill");
+  MFE_CHECK(ctx, hipStreamSynchronize(s));  // roff/nu host copies leave scope
+  A->sellx_ok = true;
+  A->sellx_maxu = maxu;
+  A->sellx_ntiles = nt;
+  A->sellx_runs = runs;
+  return MFE_OK;
+}
+
 // CSR FP32 copy of the values (round to nearest).
 __global__ void vals_to_f32_kernel(int64_t nnz, const double* __restrict__ v,
                                    float* __restrict__ v32) {
@@ -264,7 +575,8 @@ static int ensure_plan(mfe_ctx* ctx, const mfe_csr* Ac, bool need64, bool need32
   mfe_csr* A = const_cast<mfe_csr*>(Ac);
   hipStream_t s = ctx->stream;
   const int64_t n = A->nrows;
-  if (A->spmv_format == MFE_SPMV_AUTO && !A->sell_struct_valid && n > 0) {
+  const bool sell_format = A->spmv_format == MFE_SPMV_AUTO || A->spmv_format == MFE_SPMV_SELL;
+  if (sell_format && !A->sell_struct_valid && n > 0) {
     sell_release(A);
     const int64_t ns = cdiv(n, kSlice);
     int* d_width = nullptr;
@@ -310,7 +622,11 @@ static int ensure_plan(mfe_ctx* ctx, const mfe_csr* Ac, bool need64, bool need32
       sell_disable(A);
     }
   }
-  const bool sell = A->spmv_format == MFE_SPMV_AUTO && A->sell_ok && n > 0;
+  if (A->spmv_format == MFE_SPMV_AUTO && A->sell_ok && !A->sellx_valid && n > 0) {
+    int st;
+    if ((st = build_sellx(ctx, A))) return st;
+  }
+  const bool sell = sell_format && A->sell_ok && n > 0;
   const unsigned fg = (unsigned)cdiv(A->sell_nslices * kSlice, kBlock);
   if (sell && need64 && !A->sell_vals_valid) {
     if (!A->sell_vals) {
@@ -369,11 +685,16 @@ struct SpmvOp {
   const unsigned char* slen = nullptr;
   const int* scols = nullptr;
   const T* svals = nullptr;  // non-null: SELL
+  const unsigned short* xl = nullptr;  // non-null: SELL with staged x
+  const int *xroff = nullptr, *xrcol = nullptr, *xrpos = nullptr, *xnu = nullptr;
+  int xmaxu = 0;
   const void* key() const { return svals ? (const void*)svals : (const void*)vals; }
 };
 
+// allow_sell false: the CSR kernels; allow_stage false: plain SELL even where
+// the staged-x plan exists.
 template <typename T>
-static SpmvOp<T> spmv_op(const mfe_csr* A, bool allow_sell = true) {
+static SpmvOp<T> spmv_op(const mfe_csr* A, bool allow_sell = true, bool allow_stage = true) {
   SpmvOp<T> op;
   op.A = A;
   op.gen = A->plan_gen;
@@ -381,7 +702,8 @@ static SpmvOp<T> spmv_op(const mfe_csr* A, bool allow_sell = true) {
   op.ro = A->row_offsets;
   op.cols = A->cols;
   op.stream_ok = A->max_block_nnz >= 0 && A->max_block_nnz <= kSpmvMaxNnz;
-  const bool sell = allow_sell && A->spmv_format == MFE_SPMV_AUTO && A->sell_ok;
+  const bool sell = allow_sell && A->sell_ok &&
+                    (A->spmv_format == MFE_SPMV_AUTO || A->spmv_format == MFE_SPMV_SELL);
   if constexpr (std::is_same<T, double>::value) {
     op.vals = A->vals;
     if (sell && A->sell_vals_valid) op.svals = A->sell_vals;
@@ -394,6 +716,14 @@ static SpmvOp<T> spmv_op(const mfe_csr* A, bool allow_sell = true) {
     op.soff = A->sell_off;
     op.slen = A->sell_len;
     op.scols = A->sell_cols;
+    if (allow_stage && A->spmv_format == MFE_SPMV_AUTO && A->sellx_ok) {
+      op.xl = A->sellx_lcols;
+      op.xroff = A->sellx_roff;
+      op.xrcol = A->sellx_rcol;
+      op.xrpos = A->sellx_rpos;
+      op.xnu = A->sellx_nu;
+      op.xmaxu = A->sellx_maxu;
+    }
   }
   return op;
 }
@@ -409,7 +739,12 @@ static int launch_spmv(const SpmvOp<T>& op, const T* x, T* y, const T* bvec, T* 
     return 0;
   }
   const int tiles = (int)cdiv(n, kSpmvRows);
-  if (op.svals) {
+  if (op.xl) {
+    const size_t lds = sizeof(T) * (size_t)std::max(op.xmaxu, 1);  // padding slots read entry 0
+    spmv_sellx_kernel<T, ZERO_Y, MODE><<<tiles, kBlock, lds, stream>>>(
+        n, op.nslices, op.soff, op.slen, op.xl, op.svals, op.xroff, op.xrcol, op.xrpos, op.xnu, x,
+        y, bvec, partials, done);
+  } else if (op.svals) {
     spmv_sell_kernel<T, ZERO_Y, MODE><<<tiles, kBlock, 0, stream>>>(
         n, op.nslices, op.soff, op.slen, op.scols, op.svals, x, y, bvec, partials, done);
   } else if (op.stream_ok) {
@@ -1237,8 +1572,9 @@ static int read_scalar(mfe_ctx* ctx, const double* d, double* out) {
   return MFE_OK;
 }
 
-int mfe_spmv_plan(mfe_ctx* ctx, mfe_csr* A, int format, int64_t info[4]) {
-  if (!ctx || !A || (format != MFE_SPMV_AUTO && format != MFE_SPMV_CSR)) return MFE_BAD_ARG;
+int mfe_spmv_plan(mfe_ctx* ctx, mfe_csr* A, int format, int64_t info[6]) {
+  if (!ctx || !A || (format != MFE_SPMV_AUTO && format != MFE_SPMV_CSR && format != MFE_SPMV_SELL))
+    return MFE_BAD_ARG;
   MFE_CHECK(ctx, hipSetDevice(ctx->device));
   if (format != A->spmv_format) {
     MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1249,7 +1585,7 @@ int mfe_spmv_plan(mfe_ctx* ctx, mfe_csr* A, int format, int64_t info[4]) {
   if ((st = ensure_plan(ctx, A, true, false))) return st;
   if (info) {
     const SpmvOp<double> op = spmv_op<double>(A);
-    info[0] = op.svals ? 1 : (op.stream_ok ? 0 : 2);
+    info[0] = op.xl ? 3 : op.svals ? 1 : (op.stream_ok ? 0 : 2);
     info[1] = op.svals ? A->sell_slots : A->nnz;
     info[2] = op.svals ? A->sell_nslices : cdiv(A->nrows, kSpmvRows);
     int64_t longest = A->sell_max_len;
@@ -1262,6 +1598,8 @@ int mfe_spmv_plan(mfe_ctx* ctx, mfe_csr* A, int format, int64_t info[4]) {
       for (int64_t r = 0; r < A->nrows; ++r) longest = std::max<int64_t>(longest, ro[r + 1] - ro[r]);
     }
     info[3] = longest;
+    info[4] = op.xl ? A->sellx_runs : 0;
+    info[5] = op.xl ? A->sellx_maxu : 0;
   }
   MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return MFE_OK;
@@ -1323,7 +1661,7 @@ int mfe_matvec_fp32(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y
 
 int mfe_time_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y, int variant,
                     int reps, double* avg_ms) {
-  if (!ctx || !A || !avg_ms || reps <= 0 || variant < 0 || variant > 5) return MFE_BAD_ARG;
+  if (!ctx || !A || !avg_ms || reps <= 0 || variant < 0 || variant > 6) return MFE_BAD_ARG;
   int st;
   if ((st = check_vec(ctx, x, A->nrows, "matvec x")) || (st = check_vec(ctx, y, A->nrows, "matvec y")))
     return st;
@@ -1340,7 +1678,8 @@ int mfe_time_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y
     done = d;
   }
   if ((st = ensure_plan(ctx, A, !f32, f32))) return st;
-  SpmvOp<double> op64 = spmv_op<double>(A, !csr_only);
+  // variant 6: variant 1 on the SELL kernel without the staged x
+  SpmvOp<double> op64 = spmv_op<double>(A, !csr_only, variant != 6);
   SpmvOp<float> op32 = spmv_op<float>(A, !csr_only);
   CgWorkspace* w = nullptr;
   if (f32) {

@@ -242,9 +242,14 @@ void impose_dirichlet(typename MatrixType::ScalarType prescribed_value, MatrixTy
         "impose_dirichlet");
 }
 
+// The reference renumbers columns for its halo exchange here, the step
+// before the solve that prepares A for matvec; on one rank that renumbering is
+// the identity, and the SpMV storage plan (SELL-64 copy, staged-x runs) is
+// built here instead of inside the first cg_solve.
 template <typename MatrixType>
 void make_local_matrix(MatrixType& A) {
   check(mfe_make_local_matrix(context(), A.h), "make_local_matrix");
+  check(mfe_spmv_plan(context(), A.h, MFE_SPMV_AUTO, nullptr), "make_local_matrix (SpMV plan)");
   A.has_local_indices = true;
 }
 

@@ -1,11 +1,14 @@
 """GPU many right-hand sides: blocked SpMM and simultaneous CG.
 
-Checks: every SpMM tile gives, in column k, the single-vector SpMV's bits
-(the oracle's matvec) for nrhs = 4, 8, 16 on stencil and ragged matrices;
-the autotuner picks one of its timed tiles; the multi-RHS CG matches the
-oracle's per-column CG on a small mesh, and on the 128^3 workload (16
-right-hand sides, N(0,1) per free node, seed 1, zero on Dirichlet rows)
-every column reaches 1e-10 with the residual recomputed from the solution.
+Checks: every SpMM tile (direct and LDS-staged) gives, in column k, the
+single-vector SpMV's bits (the oracle's matvec) for nrhs = 4, 8, 16 on
+stencil and ragged matrices; a staged tile whose plan does not fit (a row
+tile over 1024 non-zeros) is reported unavailable and refused; the
+autotuner picks one of its timed tiles; the multi-RHS CG matches the
+oracle's per-column CG on a small mesh with the tuned tile and with each
+staged tile pinned, and on the 128^3 workload (16 right-hand sides, N(0,1)
+per free node, seed 1, zero on Dirichlet rows) every column reaches 1e-10
+with the residual recomputed from the solution.
 """
 import sys
 
@@ -44,26 +47,52 @@ def test_spmm_every_tile_bit_exact(mfe, oracle_mod):
             tun = mfe.spmm_tune(A, nrhs)
             say(f"spmm {dims} nrhs={nrhs}: tuned tile {tun['tile']} of {len(tun['ms'])}")
             assert 0 <= tun["tile"] < len(tun["ms"])
+            assert all(np.isfinite(tun["ms"])), tun["ms"]  # 27-point rows: every plan fits
             for tile in list(range(len(tun["ms"]))) + [-1]:
                 mfe.spmm(A, Xd, Yd, nrhs, tile)
                 assert bits_equal(Yd.to_numpy().reshape(n, nrhs), want), (dims, nrhs, tile)
 
 
-def test_spmm_ragged_rows(mfe, oracle_mod):
-    rng = np.random.default_rng(9)
-    n = 777
-    lens = rng.integers(0, 40, n)  # includes rows longer than 27
+def ragged_matrix(oracle_mod, rng, lens, n):
     ro = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
     cols = np.concatenate([np.sort(rng.choice(n, size=int(l), replace=False)) for l in lens]).astype(np.int32)
     vals = rng.standard_normal(cols.shape[0])
-    Ao = oracle_mod.CSR(np.arange(n, dtype=np.int32), ro, cols, vals)
-    A = mfe.CSRMatrix.from_host(ro, cols, vals)
-    X = rng.standard_normal((n, 8))
-    want = np.stack([oracle_mod.matvec(Ao, np.ascontiguousarray(X[:, k])) for k in range(8)], axis=1)
-    Xd, Yd = mfe.Vector.from_numpy(X.ravel()), mfe.Vector(n * 8)
-    for tile in range(len(mfe.spmm_tune(A, 8)["ms"])):
-        mfe.spmm(A, Xd, Yd, 8, tile)
-        assert bits_equal(Yd.to_numpy().reshape(n, 8), want), tile
+    return oracle_mod.CSR(np.arange(n, dtype=np.int32), ro, cols, vals)
+
+
+def test_spmm_ragged_rows(mfe, oracle_mod):
+    rng = np.random.default_rng(9)
+    n = 777
+    # rows of 0..39 entries (longer than 27), n not a multiple of 16 or 32:
+    # a 32-row tile can pass 1024 non-zeros, a 16-row tile cannot
+    lens = rng.integers(0, 40, n)
+    lens[32:64] = 39  # a 32-row tile of 1248 non-zeros
+    # then two rows of 700 entries in one 16-row tile: no staged plan fits
+    lens2 = lens.copy()
+    lens2[96:98] = 700
+    for lens_case, staged16 in ((lens, True), (lens2, False)):
+        Ao = ragged_matrix(oracle_mod, rng, lens_case, n)
+        A = mfe.CSRMatrix.from_host(Ao.row_offsets, Ao.cols, Ao.vals)
+        for nrhs in (4, 8, 16):
+            X = rng.standard_normal((n, nrhs))
+            want = np.stack([oracle_mod.matvec(Ao, np.ascontiguousarray(X[:, k]))
+                             for k in range(nrhs)], axis=1)
+            Xd, Yd = mfe.Vector.from_numpy(X.ravel()), mfe.Vector(n * nrhs)
+            tun = mfe.spmm_tune(A, nrhs)
+            say(f"ragged nrhs={nrhs} max len {lens_case.max()}: ms {np.round(tun['ms'], 4).tolist()}")
+            for tile, (ms, shape) in enumerate(zip(tun["ms"], tun["tiles"])):
+                if not shape["staged"]:
+                    assert np.isfinite(ms), (tile, shape)
+                elif shape["rows_per_block"] == 32 or not staged16:
+                    assert not np.isfinite(ms), (tile, shape)
+                elif nrhs <= 8:  # 16 rows x 39 entries of 8 doubles fit in LDS
+                    assert np.isfinite(ms), (tile, shape)
+                if not np.isfinite(ms):
+                    with pytest.raises(RuntimeError):
+                        mfe.spmm(A, Xd, Yd, nrhs, tile)
+                    continue
+                mfe.spmm(A, Xd, Yd, nrhs, tile)
+                assert bits_equal(Yd.to_numpy().reshape(n, nrhs), want), (nrhs, tile)
 
 
 def rhs_block(oracle_mod, dims, nrhs, seed):
@@ -98,6 +127,36 @@ def test_multi_cg_matches_oracle(mfe, oracle_mod):
             res = bk - oracle_mod.matvec(Ao, np.ascontiguousarray(Xh[:, k]))
             assert np.linalg.norm(res) <= 4e-10 * bn
             assert np.max(np.abs(Xh[:, k] - ref["x"])) <= 1e-8 * np.max(np.abs(ref["x"]))
+
+
+def test_multi_cg_staged_tiles(mfe, oracle_mod):
+    """Each staged tile pinned in turn: the CG still matches the oracle's."""
+    dims = (13, 9, 10)
+    P = oracle_mod.build_problem(*dims)
+    Ao = P["A"]
+    A = mfe.CSRMatrix.from_host(Ao.row_offsets, Ao.cols, Ao.vals)
+    mfe.make_local_matrix(A)
+    n = Ao.nrows
+    for nrhs in (4, 8, 16):
+        B = rhs_block(oracle_mod, dims, nrhs, 2)
+        refs = []
+        for k in range(nrhs):
+            bk = np.ascontiguousarray(B[:, k])
+            refs.append(oracle_mod.cg_solve(Ao, bk, max_iter=3000,
+                                            tol=1e-10 * float(np.linalg.norm(bk))))
+        for tile, shape in enumerate(mfe.spmm_tiles(nrhs)):
+            if not shape["staged"]:
+                continue
+            say(f"multi cg staged {dims} nrhs={nrhs} tile {tile} {shape}")
+            mfe.spmm_select(A, nrhs, tile)
+            X = mfe.Vector(n * nrhs)
+            r = mfe.cg_solve_multi(A, mfe.Vector.from_numpy(B.ravel()), X, nrhs, max_iter=3000,
+                                   rel_tol=1e-10)
+            Xh = X.to_numpy().reshape(n, nrhs)
+            assert np.all(r.rel_residual <= 1e-10)
+            for k, ref in enumerate(refs):
+                assert abs(int(r.iters[k]) - ref["num_iters"]) <= 2, (tile, k, r.iters[k])
+                assert np.max(np.abs(Xh[:, k] - ref["x"])) <= 1e-8 * np.max(np.abs(ref["x"]))
 
 
 def test_multi_rhs_128(mfe, oracle_mod):

@@ -220,6 +220,26 @@ def test_long_rows_take_the_rowwise_path(mfe, ctx, oracle_mod):
     assert bits_equal(y.to_numpy(), oracle_mod.matvec(Ao, xh))
 
 
+def staged_plan_model(ro, cols, n, rows=256):
+    """The staged-x plan's figures from the CSR arrays (numpy): total runs of
+    consecutive distinct columns over all 256-row tiles, the most runs and the
+    most distinct columns of any tile."""
+    total, maxr, maxu = 0, 0, 0
+    for t0 in range(0, n, rows):
+        c = np.unique(cols[ro[t0]:ro[min(t0 + rows, n)]])
+        nr = 0 if c.size == 0 else 1 + int(np.count_nonzero(np.diff(c) != 1))
+        total += nr
+        maxr, maxu = max(maxr, nr), max(maxu, int(c.size))
+    return total, maxr, maxu
+
+
+def expected_auto_kernel(ro, cols, n):
+    if n == 0 or int(np.diff(ro).max()) > 27:
+        return None
+    _, maxr, _ = staged_plan_model(ro, cols, n)
+    return "sell64-staged-x" if maxr <= 128 else "sell64"
+
+
 def _ragged_csr(rng, n, maxlen=27):
     """Random CSR with row lengths 0..maxlen (some empty), sorted columns."""
     lens = rng.integers(0, maxlen + 1, n)
@@ -232,11 +252,13 @@ def _ragged_csr(rng, n, maxlen=27):
     return ro, cols, vals
 
 
-@pytest.mark.parametrize("n", [1, 63, 64, 65, 1000, 4099])
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 257, 1000, 4099])
 def test_sell_and_csr_kernels_bit_exact_on_ragged_rows(mfe, ctx, oracle_mod, n):
-    """Both SpMV formats equal the oracle bit for bit on ragged rows, empty
-    rows (whose result is 0*y: -0.0 for negative y) and row counts that are
-    not a multiple of the 64-row slice."""
+    """Every SpMV format (SELL-64 with staged x where the plan fits, plain
+    SELL-64, CSR) equals the oracle bit for bit on ragged rows, empty rows
+    (whose result is 0*y: -0.0 for negative y) and row counts that are not a
+    multiple of the 64-row slice or the 256-row tile; 4099 rows of random
+    columns have too many runs per tile and stay on plain SELL-64."""
     say(f"case ragged spmv n={n}")
     rng = np.random.default_rng(n)
     ro, cols, vals = _ragged_csr(rng, n)
@@ -245,10 +267,16 @@ def test_sell_and_csr_kernels_bit_exact_on_ragged_rows(mfe, ctx, oracle_mod, n):
     yh = -np.abs(rng.standard_normal(n)) - 1.0
     want = oracle_mod.matvec(Ao, xh, yh.copy())
     want32 = oracle_mod.matvec_f32(Ao, xh)
-    for fmt, kernel in [(mfe.SPMV_AUTO, "sell64"), (mfe.SPMV_CSR, "csr-stream")]:
+    auto = expected_auto_kernel(ro, cols, n)
+    assert auto == ("sell64" if n == 4099 else "sell64-staged-x")
+    for fmt, kernel in [(mfe.SPMV_AUTO, auto), (mfe.SPMV_SELL, "sell64"),
+                        (mfe.SPMV_CSR, "csr-stream")]:
         A = mfe.CSRMatrix.from_host(ro, cols, vals, ctx=ctx)
         info = mfe.spmv_plan(A, fmt)
         assert info["kernel"] == kernel and info["longest_row"] == int(np.diff(ro).max())
+        if kernel == "sell64-staged-x":
+            runs, _, maxu = staged_plan_model(ro, cols, n)
+            assert (info["runs"], info["max_distinct"]) == (runs, maxu)
         x = mfe.Vector.from_numpy(xh, ctx)
         y = mfe.Vector.from_numpy(yh, ctx)
         mfe.matvec(A, x, y)
@@ -272,15 +300,16 @@ def test_sell_uniform_slice_widths(mfe, ctx, oracle_mod, width):
                            for _ in range(n)]).astype(np.int32)
     vals = rng.standard_normal(cols.shape[0])
     Ao = oracle_mod.CSR(np.arange(n, dtype=np.int32), ro, cols, vals)
-    A = mfe.CSRMatrix.from_host(ro, cols, vals, ctx=ctx)
-    assert mfe.spmv_plan(A)["kernel"] == "sell64"
     xh = rng.standard_normal(n)
-    x = mfe.Vector.from_numpy(xh, ctx)
-    y = mfe.Vector(n, ctx)
-    mfe.matvec(A, x, y)
-    assert bits_equal(y.to_numpy(), oracle_mod.matvec(Ao, xh))
-    mfe.matvec_fp32(A, x, y)
-    assert bits_equal(y.to_numpy(), oracle_mod.matvec_f32(Ao, xh))
+    for fmt, kernel in [(mfe.SPMV_AUTO, "sell64-staged-x"), (mfe.SPMV_SELL, "sell64")]:
+        A = mfe.CSRMatrix.from_host(ro, cols, vals, ctx=ctx)
+        assert mfe.spmv_plan(A, fmt)["kernel"] == kernel
+        x = mfe.Vector.from_numpy(xh, ctx)
+        y = mfe.Vector(n, ctx)
+        mfe.matvec(A, x, y)
+        assert bits_equal(y.to_numpy(), oracle_mod.matvec(Ao, xh)), kernel
+        mfe.matvec_fp32(A, x, y)
+        assert bits_equal(y.to_numpy(), oracle_mod.matvec_f32(Ao, xh)), kernel
 
 
 def test_sell_plan_follows_value_and_structure_changes(mfe, ctx, oracle_mod):
@@ -293,7 +322,7 @@ def test_sell_plan_follows_value_and_structure_changes(mfe, ctx, oracle_mod):
     xh = np.random.default_rng(2).standard_normal(A.nrows)
     x = mfe.Vector.from_numpy(xh, ctx)
     y = mfe.Vector(A.nrows, ctx)
-    assert mfe.spmv_plan(A)["kernel"] == "sell64"
+    assert mfe.spmv_plan(A)["kernel"] == "sell64-staged-x"
     mfe.matvec(A, x, y)
     assert bits_equal(y.to_numpy(), oracle_mod.matvec(P["A"], xh))
     mfe.impose_dirichlet(0.0, A, b, dims[0] + 1, dims[1] + 1, dims[2] + 1, [3, 4, 50])
@@ -313,18 +342,57 @@ def test_sell_plan_follows_value_and_structure_changes(mfe, ctx, oracle_mod):
 
 
 def test_cg_identical_across_spmv_formats(mfe, ctx):
-    """Both formats produce the same per-256-row partials, so the fast CG
+    """Every format produces the same per-256-row partials, so the fast CG
     trajectory and solution are bit-identical."""
     say("case cg formats")
     out = []
-    for fmt in (mfe.SPMV_CSR, mfe.SPMV_AUTO):
+    for fmt in (mfe.SPMV_CSR, mfe.SPMV_SELL, mfe.SPMV_AUTO):
         mesh, A, b, _ = build_stages(mfe, ctx, (24, 20, 22))
         mfe.spmv_plan(A, fmt)
         x = mfe.Vector(A.nrows, ctx)
         r = mfe.cg_solve(A, b, x, max_iter=60, tolerance=0.0, history=True)
         out.append((x.to_numpy(), r.history))
-    assert bits_equal(out[0][0], out[1][0])
-    assert bits_equal(out[0][1], out[1][1])
+    for other in out[1:]:
+        assert bits_equal(out[0][0], other[0])
+        assert bits_equal(out[0][1], other[1])
+
+
+@pytest.mark.parametrize("band,kernel", [(300, "sell64-staged-x"), (None, "sell64")])
+def test_staged_x_on_banded_and_scattered_columns(mfe, ctx, oracle_mod, band, kernel):
+    """20,000 rows of 1..27 random columns: within +-300 of the row (irregular
+    runs, staged x; the plan's run and distinct counts match a numpy model),
+    or anywhere (too many runs per tile: plain SELL-64).  SpMV, SpMV+dot and
+    the FP32 SpMV equal the oracle bit for bit."""
+    say(f"case staged x band={band}")
+    rng = np.random.default_rng(17)
+    n = 20000
+    lens = rng.integers(1, 28, n)
+    parts = []
+    for r, l in enumerate(lens):
+        lo, hi = (max(0, r - band), min(n, r + band + 1)) if band else (0, n)
+        parts.append(np.sort(rng.choice(np.arange(lo, hi), size=int(l), replace=False)))
+    ro = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    cols = np.concatenate(parts).astype(np.int32)
+    vals = rng.standard_normal(cols.shape[0])
+    assert expected_auto_kernel(ro, cols, n) == kernel
+    Ao = oracle_mod.CSR(np.arange(n, dtype=np.int32), ro, cols, vals)
+    A = mfe.CSRMatrix.from_host(ro, cols, vals, ctx=ctx)
+    info = mfe.spmv_plan(A)
+    assert info["kernel"] == kernel
+    if kernel == "sell64-staged-x":
+        runs, _, maxu = staged_plan_model(ro, cols, n)
+        assert (info["runs"], info["max_distinct"]) == (runs, maxu)
+    xh = rng.standard_normal(n)
+    x = mfe.Vector.from_numpy(xh, ctx)
+    y = mfe.Vector(n, ctx)
+    mfe.matvec(A, x, y)
+    want = oracle_mod.matvec(Ao, xh)
+    assert bits_equal(y.to_numpy(), want)
+    d = mfe.matvec_and_dot(A, x, y)
+    assert bits_equal(y.to_numpy(), want)
+    assert abs(d - float(xh @ want)) <= 1e-12 * float(np.abs(xh * want).sum())
+    mfe.matvec_fp32(A, x, y)
+    assert bits_equal(y.to_numpy(), oracle_mod.matvec_f32(Ao, xh))
 
 
 @pytest.mark.parametrize("dims", [(10, 10, 10), (40, 33, 29)])
