@@ -429,6 +429,12 @@ def main():
                      "traffic": measured_traffic(N, kname + "<double"),
                      "kernel": kname + "<double,false,kDot> (CG SpMV + p.Ap partials)",
                      "bytes_per_launch": byt, "avg_ms": ms_cg},
+        # the CG iteration against the fused minimum per iteration: one CSR SpMV
+        # (12 nnz + 4(n+1) + 16n) plus x, p, r, Ap read and x, r, p written (56n)
+        "cg_iteration": {"fused_min_bytes": spmv_bytes(n, nnz) + 56 * n,
+                         "GBs": it_s * (spmv_bytes(n, nnz) + 56 * n) / 1e9,
+                         "frac_of_peak": it_s * (spmv_bytes(n, nnz) + 56 * n) / 1e9 / HBM_PEAK_GBS,
+                         "ms": 1e3 / it_s},
         "spmv_GBs": gbs_cg,
         "spmv_std_GBs": gbs_std,
         "spmv_fp32_GBs": gbs_32,
