@@ -293,13 +293,15 @@ int mfe_spmm_tune(mfe_ctx* ctx, const mfe_csr* A, int nrhs, int* tile, double* m
  * Kept until A's structure changes or another matrix is tuned on ctx. */
 int mfe_spmm_select(mfe_ctx* ctx, const mfe_csr* A, int nrhs, int tile);
 /* Shape of a tile: columns per lane, non-zeros per unrolled chunk, rows per
- * workgroup, and staged = 1 for tiles that copy the distinct X rows of 32
- * consecutive rows into LDS before the products (available when no 32-row
- * tile has more than 1024 non-zeros and its X rows fit 64 KiB; the tuner
- * reports ms = +inf for an unavailable tile, and mfe_spmm rejects it).
- * Outputs nullable. */
+ * workgroup, and kind: 0 = CSR arrays, each lane gathering its X pieces;
+ * 1 = staged, the distinct X rows of 16 or 32 consecutive rows copied into
+ * LDS first (available when no such row tile has more than 1024 non-zeros and
+ * its X rows fit 64 KiB); 2 = SELL, values and columns read from A's SELL-64
+ * copy (available unless A's format is CSR-only or a row is longer than 27).
+ * The tuner reports ms = +inf for an unavailable tile, and mfe_spmm rejects
+ * it.  Outputs nullable. */
 int mfe_spmm_tile_info(int nrhs, int tile, int* cols_per_lane, int* unroll, int* rows_per_block,
-                       int* staged);
+                       int* kind);
 /* nrhs simultaneous FP64 CG solves A X[:, k] = B[:, k] from the given X,
  * sharing one SpMM per iteration; column k stops once ||r_k|| <= rel_tol *
  * ||B[:, k]|| (cg_solve.hpp's loop test per column) or after max_iter.

@@ -622,8 +622,9 @@ def spmm_tiles(nrhs: int) -> list:
         if lib().mfe_spmm_tile_info(int(nrhs), i, ctypes.byref(c), ctypes.byref(u),
                                     ctypes.byref(r), ctypes.byref(st)) != 0:
             return tiles
+        kind = ("csr", "staged", "sell")[st.value]
         tiles.append({"cols_per_lane": c.value, "unroll": u.value, "rows_per_block": r.value,
-                      "staged": bool(st.value)})
+                      "kind": kind, "staged": kind == "staged"})
         i += 1
 
 

@@ -123,6 +123,10 @@ inline void csr_structure_changed(mfe_csr* A) {
 }
 // Frees the SELL arrays (mfe_solve.hip).
 void sell_release(mfe_csr* A);
+// Builds A's SELL-64 copy with FP64 values if its format uses one and it is
+// stale (mfe_solve.hip; synchronises the stream, not for graph capture).
+// Leaves A->sell_ok false when the format is CSR or SELL does not apply.
+int sell_prepare(mfe_ctx* ctx, const mfe_csr* A);
 
 }  // namespace mfe
 

@@ -9,8 +9,11 @@
 // value/column entry (a broadcast).  The row is walked in chunks of U entries
 // whose loads are all issued before the first product; each column's sum runs
 // in stored column order, so column k of Y equals A * X[:, k] from the
-// single-vector SpMV bit for bit.  Staged tiles first copy the distinct X rows
-// of 32 consecutive rows into LDS and gather from there (same arithmetic).
+// single-vector SpMV bit for bit.  SELL tiles read the values and columns
+// from the matrix's SELL-64 copy (slot-major: the loads of a wave's rows are
+// contiguous) instead of the CSR arrays.  Staged tiles first copy the distinct
+// X rows of 16 or 32 consecutive rows into LDS and gather from there (same
+// arithmetic).
 // The autotuner times every tile on the caller's matrix and keeps the fastest
 // for that matrix and K.
 #include <algorithm>
@@ -18,22 +21,30 @@
 #include <vector>
 
 #include "mfe_common.hpp"
+#include "mfe_sell.hpp"
 
 namespace mfe {
 
+constexpr int kSellTile = -1;  // SpmmTile::rows of a tile reading the SELL-64 copy
+
 struct SpmmTile {
   int cpl, unroll;
-  int rows;  // 0: direct gathers; 16 or 32: rows per workgroup of a staged tile
+  int rows;  // 0: CSR, direct gathers; 16 or 32: rows per workgroup of a staged tile;
+             // kSellTile: SELL-64 copy, direct gathers
 };
 
 // Candidate tiles per K (index = tile id).
 static const SpmmTile kTiles16[] = {{1, 9, 0},  {2, 9, 0},  {4, 9, 0},  {8, 9, 0},  {2, 3, 0},
                                     {4, 3, 0},  {8, 3, 0},  {16, 3, 0}, {1, 9, 16}, {2, 9, 16},
-                                    {1, 9, 32}, {2, 9, 32}, {4, 9, 32}};
-static const SpmmTile kTiles8[] = {{1, 9, 0},  {2, 9, 0},  {4, 9, 0},  {8, 9, 0},  {2, 3, 0},
-                                   {4, 3, 0},  {8, 3, 0},  {1, 9, 16}, {1, 9, 32}, {2, 9, 32}};
-static const SpmmTile kTiles4[] = {{1, 9, 0}, {2, 9, 0},  {4, 9, 0},
-                                   {2, 3, 0}, {4, 3, 0},  {1, 9, 16}, {1, 9, 32}};
+                                    {1, 9, 32}, {2, 9, 32}, {4, 9, 32}, {2, 9, kSellTile},
+                                    {4, 9, kSellTile}, {8, 9, kSellTile}};
+static const SpmmTile kTiles8[] = {{1, 9, 0},  {2, 9, 0},  {4, 9, 0},         {8, 9, 0},
+                                   {2, 3, 0},  {4, 3, 0},  {8, 3, 0},         {1, 9, 16},
+                                   {1, 9, 32}, {2, 9, 32}, {1, 9, kSellTile}, {2, 9, kSellTile},
+                                   {4, 9, kSellTile}};
+static const SpmmTile kTiles4[] = {{1, 9, 0},  {2, 9, 0},  {4, 9, 0},         {2, 3, 0},
+                                   {4, 3, 0},  {1, 9, 16}, {1, 9, 32},        {1, 9, kSellTile},
+                                   {2, 9, kSellTile}};
 constexpr int kMaxTiles = 16;
 
 static int tile_count(int K) {
@@ -389,17 +400,106 @@ static int build_stage_plan(mfe_ctx* ctx, const mfe_csr* A, int R, SpmmPlan& p) 
 }
 
 // Operands of one SpMM launch; plans[plan_slot(R)] is the R-row staging plan
-// (null: staged tiles of that R unavailable).
+// (null: staged tiles of that R unavailable); svals null: no SELL tiles.
 struct SpmmArgs {
   int n;
   const int* ro;
   const int* cols;
   const double* vals;
   const SpmmPlan* plans[2];
+  int nslices;
+  const int* soff;
+  const unsigned char* slen;
+  const int* scols;
+  const double* svals;
 };
 
 static bool stage_ok(const SpmmPlan* p, int K) {
   return p && p->ok && (int64_t)p->maxu * K * (int64_t)sizeof(double) <= kStageLdsBytes;
+}
+
+// spmm_kernel on the SELL-64 copy: thread (row, column group) reads slot j of
+// its row at sell_pos (the L lanes of a row read the same entry, the rows of a
+// wave consecutive entries); RPB rows per workgroup cover whole or parts of
+// slices; rows past the last slice return at once, rows past n add nothing.
+template <int K, int CPL, int U>
+__global__ __launch_bounds__(kBlock) void spmm_sell_kernel(
+    int n, int nslices, const int* __restrict__ soff, const unsigned char* __restrict__ slen,
+    const int* __restrict__ scols, const double* __restrict__ svals, const double* __restrict__ X,
+    double* __restrict__ Y, double* __restrict__ partials, const int* __restrict__ done) {
+  if (done && *done) return;
+  constexpr int L = K / CPL;
+  constexpr int RPB = kBlock / L;
+  const int t = (int)threadIdx.x;
+  const int row = (int)blockIdx.x * RPB + t / L;
+  const int k0 = (t % L) * CPL;
+  const int slice = row / kWave, lane = row % kWave;
+  const bool live = slice < nslices;
+  double acc[CPL];
+#pragma unroll
+  for (int q = 0; q < CPL; ++q) acc[q] = 0.0;
+  if (live) {
+    const int o = soff[slice], w = soff[slice + 1] - o;
+    const int len = slen[row];
+    const int64_t o64 = (int64_t)o * kWave;
+    for (int j0 = 0; j0 < len; j0 += U) {
+      int c[U];
+      double v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t at = sell_pos(o64, min(j0 + u, w - 1), w, lane);
+        c[u] = scols[at];
+        v[u] = svals[at];
+      }
+      double xv[U][CPL];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const double* src = X + (int64_t)c[u] * K + k0;
+        if constexpr (CPL % 2 == 0) {
+#pragma unroll
+          for (int q = 0; q < CPL; q += 2) {
+            const double2 d = *reinterpret_cast<const double2*>(src + q);
+            xv[u][q] = d.x;
+            xv[u][q + 1] = d.y;
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < CPL; ++q) xv[u][q] = src[q];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (j0 + u < len) {
+#pragma unroll
+          for (int q = 0; q < CPL; ++q) acc[q] += v[u] * xv[u][q];
+        }
+    }
+    if (row < n) {
+      double* dst = Y + (int64_t)row * K + k0;
+#pragma unroll
+      for (int q = 0; q < CPL; ++q) dst[q] = acc[q];
+    }
+  }
+  if (partials) {
+    __shared__ double red[kBlock / kWave][K];
+    double term[CPL];
+#pragma unroll
+    for (int q = 0; q < CPL; ++q)
+      term[q] = row < n ? acc[q] * X[(int64_t)row * K + k0 + q] : 0.0;
+    const int lw = t % kWave, wave = t / kWave;
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) {
+      const double s = sum_same_column<L>(term[q]);
+      if (lw < L) red[wave][lw * CPL + q] = s;
+    }
+    __syncthreads();
+    if (t < K) {
+      double s = red[0][t];
+#pragma unroll
+      for (int w = 1; w < kBlock / kWave; ++w) s += red[w][t];
+      partials[(int64_t)blockIdx.x * K + t] = s;
+    }
+  }
 }
 
 // Staged shapes need whole waves and at most 1024 threads: 64 <= R (K / CPL).
@@ -422,7 +522,26 @@ static hipError_t launch_spmm(int tile, const SpmmArgs& a, const double* X, doub
                               double* partials, const int* done, hipStream_t s, int* blocks_out) {
   const SpmmTile T = tile_of(K, tile);
   const int n = a.n;
-  if (T.rows) {
+  if (T.rows == kSellTile) {
+    if (!a.svals) return hipErrorInvalidValue;
+    const int rpb = kBlock / (K / T.cpl);
+    const unsigned g = (unsigned)cdiv((int64_t)a.nslices * kWave, rpb);
+    if (blocks_out) *blocks_out = (int)g;
+    if (n == 0) return hipSuccess;
+#define MFE_SELL_CASE(C, UU)                                                                 \
+    if (T.cpl == C && T.unroll == UU) {                                                     \
+      spmm_sell_kernel<K, (C <= K ? C : K), UU><<<g, kBlock, 0, s>>>(                       \
+          n, a.nslices, a.soff, a.slen, a.scols, a.svals, X, Y, partials, done);           \
+      return hipGetLastError();                                                             \
+    }
+    MFE_SELL_CASE(1, 9)
+    MFE_SELL_CASE(2, 9)
+    MFE_SELL_CASE(4, 9)
+    MFE_SELL_CASE(8, 9)
+#undef MFE_SELL_CASE
+    return hipErrorInvalidValue;
+  }
+  if (T.rows > 0) {
     const SpmmPlan* p = a.plans[plan_slot(T.rows)];
     if (!stage_ok(p, K) || p->rows != T.rows) return hipErrorInvalidValue;
     if (blocks_out) *blocks_out = p->ntiles;
@@ -473,9 +592,10 @@ static hipError_t launch_spmm_k(int K, int tile, const SpmmArgs& a, const double
 
 static int k_slot(int K) { return K == 16 ? 2 : K == 8 ? 1 : 0; }
 
-// Tile `tile` can run on A for K with these plans.
+// Tile `tile` can run on A for K with these operands.
 static bool tile_available(int K, int tile, const SpmmArgs& a) {
   const int R = tile_of(K, tile).rows;
+  if (R == kSellTile) return a.svals != nullptr && a.n > 0;
   return R == 0 || stage_ok(a.plans[plan_slot(R)], K);
 }
 
@@ -553,9 +673,12 @@ __global__ __launch_bounds__(kBlock) void reduce_cols_kernel(const double* __res
 
 // ------------------------------------------------------------ CG state -------
 
+// pending[c]: column c's update X += palpha[c] P of the last iteration is
+// still to be applied (by the next P update, or by multi_finish_kernel).
 struct MultiState {
-  double rtrans[16], oldrtrans[16], pAp[16], alpha[16], beta[16], normr[16], tol[16], bnorm[16];
-  int active[16], iters[16], breakdown[16];
+  double rtrans[16], oldrtrans[16], pAp[16], alpha[16], beta[16], normr[16], tol[16], bnorm[16],
+      palpha[16];
+  int active[16], iters[16], breakdown[16], pending[16];
   int k, max_iter, done, pad;
 };
 
@@ -576,6 +699,8 @@ __global__ void multi_init_kernel(MultiState* st, int K, int max_iter, double re
     st->alpha[c] = st->beta[c] = st->pAp[c] = 0.0;
     st->iters[c] = 0;
     st->breakdown[c] = 0;
+    st->pending[c] = 0;
+    st->palpha[c] = 0.0;
     st->active[c] = live && max_iter >= 1 && st->normr[c] > st->tol[c];
   }
   __syncthreads();
@@ -586,19 +711,37 @@ __global__ void multi_init_kernel(MultiState* st, int K, int max_iter, double re
   }
 }
 
-// P = R (k = 1) or P = R + beta P for active columns.
+// The last iteration's X += palpha P where pending, then P = R (k = 1) or
+// P = R + beta P for active columns (grid-stride with a stride that is a
+// multiple of K: a thread always sees the same column).
 template <int K>
 __global__ __launch_bounds__(kBlock) void multi_p_kernel(int64_t nk, const MultiState* st,
                                                          const double* __restrict__ R,
-                                                         double* __restrict__ P) {
+                                                         double* __restrict__ P,
+                                                         double* __restrict__ X) {
   if (st->done) return;
   const bool first = st->k == 1;
+  const int c = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) % K);
+  const bool pend = st->pending[c], act = st->active[c];
+  const double a = st->palpha[c], b = st->beta[c];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nk; i += stride) {
-    const int c = (int)(i % K);
-    if (!st->active[c]) continue;
-    P[i] = first ? 1.0 * R[i] + 0.0 * R[i] : 1.0 * R[i] + st->beta[c] * P[i];
+    if (pend) X[i] = 1.0 * X[i] + a * P[i];
+    if (act) P[i] = first ? 1.0 * R[i] + 0.0 * R[i] : 1.0 * R[i] + b * P[i];
   }
+}
+
+// The update still pending when the loop has stopped.
+template <int K>
+__global__ __launch_bounds__(kBlock) void multi_finish_kernel(int64_t nk, const MultiState* st,
+                                                              const double* __restrict__ P,
+                                                              double* __restrict__ X) {
+  const int c = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) % K);
+  if (!st->pending[c]) return;
+  const double a = st->palpha[c];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nk; i += stride)
+    X[i] = 1.0 * X[i] + a * P[i];
 }
 
 __global__ void multi_alpha_kernel(MultiState* st, int K, const double* pAp) {
@@ -614,16 +757,18 @@ __global__ void multi_alpha_kernel(MultiState* st, int K, const double* pAp) {
       st->alpha[c] = st->rtrans[c] / pAp[c];
     }
   }
+  if (c < K) {  // this iteration's X update, applied by the next P update
+    st->pending[c] = st->active[c];
+    st->palpha[c] = st->alpha[c];
+  }
 }
 
-// X += alpha P, R -= alpha AP for active columns, with R.R partials per column
+// R -= alpha AP for active columns, with R.R partials per column
 // (grid-stride: a thread always sees the same column since the stride is a
-// multiple of K).
+// multiple of K).  X += alpha P is left to the next P update.
 template <int K>
 __global__ __launch_bounds__(kBlock) void multi_update_kernel(int64_t nk, const MultiState* st,
-                                                              double* __restrict__ X,
                                                               double* __restrict__ R,
-                                                              const double* __restrict__ P,
                                                               const double* __restrict__ AP,
                                                               double* __restrict__ partials) {
   if (st->done) return;
@@ -636,7 +781,6 @@ __global__ __launch_bounds__(kBlock) void multi_update_kernel(int64_t nk, const 
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + t; i < nk; i += stride) {
     double r = R[i];
     if (act) {
-      X[i] = 1.0 * X[i] + a * P[i];
       r = 1.0 * r + (-a) * AP[i];
       R[i] = r;
     }
@@ -767,13 +911,22 @@ static int ensure_plan(mfe_ctx* ctx, const mfe_csr* A, int R) {
   return MFE_OK;
 }
 
-// Operands for A with the workspace's plans that belong to A (others null).
+// Operands for A with the workspace's plans that belong to A (others null)
+// and A's SELL-64 copy when it is current.
 static SpmmArgs spmm_args(const mfe_csr* A, const MultiWorkspace* w) {
-  SpmmArgs a{(int)A->nrows, A->row_offsets, A->cols, A->vals, {nullptr, nullptr}};
+  SpmmArgs a{(int)A->nrows, A->row_offsets, A->cols, A->vals, {nullptr, nullptr},
+             0, nullptr, nullptr, nullptr, nullptr};
   if (w)
     for (int i = 0; i < 2; ++i)
       if (w->plans[i].built && w->plans[i].A == A && w->plans[i].gen == A->plan_gen)
         a.plans[i] = &w->plans[i];
+  if (A->sell_ok && A->sell_vals_valid && A->sell_vals) {
+    a.nslices = (int)A->sell_nslices;
+    a.soff = A->sell_off;
+    a.slen = A->sell_len;
+    a.scols = A->sell_cols;
+    a.svals = A->sell_vals;
+  }
   return a;
 }
 
@@ -828,13 +981,13 @@ static hipError_t enqueue_multi_iteration(MultiWorkspace* w, const mfe_csr* A, d
   const int64_t nk = w->nk;
   const unsigned vg = multi_vec_grid(nk);
   const int* done = &w->st->done;
-  multi_p_kernel<K><<<vg, kBlock, 0, s>>>(nk, w->st, w->R, w->P);
+  multi_p_kernel<K><<<vg, kBlock, 0, s>>>(nk, w->st, w->R, w->P, X);
   int nb = 0;
   hipError_t e = launch_spmm<K>(tile, spmm_args(A, w), w->P, w->AP, w->part, done, s, &nb);
   if (e != hipSuccess) return e;
   if ((e = reduce_cols(K, w->part, nb, w->part2, w->res, done, s)) != hipSuccess) return e;
   multi_alpha_kernel<<<1, 64, 0, s>>>(w->st, K, w->res);
-  multi_update_kernel<K><<<vg, kBlock, 0, s>>>(nk, w->st, X, w->R, w->P, w->AP, w->part);
+  multi_update_kernel<K><<<vg, kBlock, 0, s>>>(nk, w->st, w->R, w->AP, w->part);
   if ((e = reduce_cols(K, w->part, (int)vg, w->part2, w->res + 16, done, s)) != hipSuccess)
     return e;
   multi_next_kernel<<<1, 64, 0, s>>>(w->st, K, w->res + 16);
@@ -848,6 +1001,16 @@ static hipError_t enqueue_multi(int K, MultiWorkspace* w, const mfe_csr* A, doub
     case 8: return enqueue_multi_iteration<8>(w, A, X, tile, s);
     default: return enqueue_multi_iteration<4>(w, A, X, tile, s);
   }
+}
+
+static hipError_t launch_multi_finish(int K, MultiWorkspace* w, double* X, hipStream_t s) {
+  const unsigned vg = multi_vec_grid(w->nk);
+  switch (K) {
+    case 16: multi_finish_kernel<16><<<vg, kBlock, 0, s>>>(w->nk, w->st, w->P, X); break;
+    case 8: multi_finish_kernel<8><<<vg, kBlock, 0, s>>>(w->nk, w->st, w->P, X); break;
+    default: multi_finish_kernel<4><<<vg, kBlock, 0, s>>>(w->nk, w->st, w->P, X);
+  }
+  return hipGetLastError();
 }
 
 static hipError_t launch_multi_dot(int K, int64_t nk, const double* a, const double* b,
@@ -866,7 +1029,9 @@ static bool k_ok(int K) { return K == 4 || K == 8 || K == 16; }
 static int ensure_tuned(mfe_ctx* ctx, const mfe_csr* A, int K, MultiWorkspace* w) {
   const int slot = k_slot(K);
   int st;
-  if ((st = ensure_plan(ctx, A, 16)) || (st = ensure_plan(ctx, A, 32))) return st;
+  if ((st = sell_prepare(ctx, A)) || (st = ensure_plan(ctx, A, 16)) ||
+      (st = ensure_plan(ctx, A, 32)))
+    return st;
   if (w->tun.A == A && w->tun.gen == A->plan_gen && w->tun.tile[slot] >= 0) return MFE_OK;
   if (w->tun.A != A || w->tun.gen != A->plan_gen) {
     w->tun = SpmmTuning();
@@ -907,7 +1072,9 @@ int mfe_spmm_select(mfe_ctx* ctx, const mfe_csr* A, int nrhs, int tile) {
   MultiWorkspace* w = nullptr;
   int st;
   if ((st = multi_prepare(ctx, A->nrows, nrhs, &w))) return st;
-  if ((st = ensure_plan(ctx, A, 16)) || (st = ensure_plan(ctx, A, 32))) return st;
+  if ((st = sell_prepare(ctx, A)) || (st = ensure_plan(ctx, A, 16)) ||
+      (st = ensure_plan(ctx, A, 32)))
+    return st;
   if (!tile_available(nrhs, tile, spmm_args(A, w)))
     return fail(ctx, MFE_BAD_ARG, "spmm_select: staged tile unavailable for this matrix");
   if (w->tun.A != A || w->tun.gen != A->plan_gen) {
@@ -920,13 +1087,13 @@ int mfe_spmm_select(mfe_ctx* ctx, const mfe_csr* A, int nrhs, int tile) {
 }
 
 int mfe_spmm_tile_info(int nrhs, int tile, int* cols_per_lane, int* unroll, int* rows_per_block,
-                       int* staged) {
+                       int* kind) {
   if (!k_ok(nrhs) || tile < 0 || tile >= tile_count(nrhs)) return MFE_BAD_ARG;
   const SpmmTile T = tile_of(nrhs, tile);
   if (cols_per_lane) *cols_per_lane = T.cpl;
   if (unroll) *unroll = T.unroll;
-  if (rows_per_block) *rows_per_block = T.rows ? T.rows : kBlock / (nrhs / T.cpl);
-  if (staged) *staged = T.rows ? 1 : 0;
+  if (rows_per_block) *rows_per_block = T.rows > 0 ? T.rows : kBlock / (nrhs / T.cpl);
+  if (kind) *kind = T.rows > 0 ? 1 : T.rows == kSellTile ? 2 : 0;
   return MFE_OK;
 }
 
@@ -944,12 +1111,18 @@ int mfe_spmm(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* X, mfe_vec* Y, int n
     tile = w->tun.tile[k_slot(nrhs)];
   }
   const int R = tile_of(nrhs, tile).rows;
-  if (R) {
+  if (R > 0) {
     if ((st = ensure_plan(ctx, A, R))) return st;
     if (!stage_ok(&ctx->multi->plans[plan_slot(R)], nrhs))
       return fail(ctx, MFE_BAD_ARG,
                   "spmm: staged tile unavailable for this matrix (a row tile has more than "
                   "1024 non-zeros or its distinct X rows exceed 64 KiB)");
+  } else if (R == kSellTile) {
+    if ((st = sell_prepare(ctx, A))) return st;
+    if (!(A->sell_ok && A->sell_vals_valid) || A->nrows == 0)
+      return fail(ctx, MFE_BAD_ARG,
+                  "spmm: SELL tile unavailable for this matrix (CSR format, or a row longer "
+                  "than 27 entries)");
   }
   MFE_CHECK(ctx, launch_spmm_k(nrhs, tile, spmm_args(A, ctx->multi), X->d, Y->d, nullptr, nullptr,
                                ctx->stream, nullptr));
@@ -1014,6 +1187,7 @@ int mfe_cg_solve_multi(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* B, mfe_vec
       MFE_CHECK(ctx, hipStreamSynchronize(s));
       if (w->h_st->done) break;
     }
+    MFE_CHECK(ctx, launch_multi_finish(K, w, X->d, s));
   }
   MFE_CHECK(ctx, hipEventRecord(ctx->t_stop, s));
   MFE_CHECK(ctx, hipEventSynchronize(ctx->t_stop));

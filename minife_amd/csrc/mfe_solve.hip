@@ -669,6 +669,8 @@ static int ensure_plan(mfe_ctx* ctx, const mfe_csr* Ac, bool need64, bool need32
   return MFE_OK;
 }
 
+int sell_prepare(mfe_ctx* ctx, const mfe_csr* A) { return ensure_plan(ctx, A, true, false); }
+
 // What one SpMV launch reads: the SELL copy when the plan has one for T,
 // else the CSR arrays (FP64 values, or the CSR FP32 copy).
 template <typename T>

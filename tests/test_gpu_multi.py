@@ -1,14 +1,16 @@
 """GPU many right-hand sides: blocked SpMM and simultaneous CG.
 
-Checks: every SpMM tile (direct and LDS-staged) gives, in column k, the
-single-vector SpMV's bits (the oracle's matvec) for nrhs = 4, 8, 16 on
-stencil and ragged matrices; a staged tile whose plan does not fit (a row
-tile over 1024 non-zeros) is reported unavailable and refused; the
-autotuner picks one of its timed tiles; the multi-RHS CG matches the
-oracle's per-column CG on a small mesh with the tuned tile and with each
-staged tile pinned, and on the 128^3 workload (16 right-hand sides, N(0,1)
-per free node, seed 1, zero on Dirichlet rows) every column reaches 1e-10
-with the residual recomputed from the solution.
+Checks: every SpMM tile (CSR, SELL-64 and LDS-staged) gives, in column k,
+the single-vector SpMV's bits (the oracle's matvec) for nrhs = 4, 8, 16 on
+stencil and ragged matrices; each tile is available exactly when its
+conditions hold (computed here with numpy: SELL rows of at most 27 entries;
+staged row tiles of at most 1024 non-zeros whose distinct X rows fit 60 KiB)
+and an unavailable one is refused; the autotuner picks one of its timed
+tiles; the multi-RHS CG matches the oracle's per-column CG on a small mesh
+with the tuned tile and with each SELL and staged tile pinned, and on the
+128^3 workload (16 right-hand sides, N(0,1) per free node, seed 1, zero on
+Dirichlet rows) every column reaches 1e-10 with the residual recomputed from
+the solution.
 """
 import sys
 
@@ -53,6 +55,23 @@ def test_spmm_every_tile_bit_exact(mfe, oracle_mod):
                 assert bits_equal(Yd.to_numpy().reshape(n, nrhs), want), (dims, nrhs, tile)
 
 
+def tile_expected_available(shape, ro, cols, n, nrhs):
+    """Whether an SpMM tile can run on this CSR matrix (numpy restatement of the
+    library's conditions)."""
+    if shape["kind"] == "csr":
+        return True
+    if shape["kind"] == "sell":
+        return n > 0 and int(np.diff(ro).max()) <= 27
+    R = shape["rows_per_block"]
+    maxu = 0
+    for t0 in range(0, n, R):
+        a, b = int(ro[t0]), int(ro[min(t0 + R, n)])
+        if b - a > 1024:
+            return False
+        maxu = max(maxu, int(np.unique(cols[a:b]).size))
+    return n > 0 and maxu * nrhs * 8 <= 60 * 1024
+
+
 def ragged_matrix(oracle_mod, rng, lens, n):
     ro = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
     cols = np.concatenate([np.sort(rng.choice(n, size=int(l), replace=False)) for l in lens]).astype(np.int32)
@@ -70,7 +89,9 @@ def test_spmm_ragged_rows(mfe, oracle_mod):
     # then two rows of 700 entries in one 16-row tile: no staged plan fits
     lens2 = lens.copy()
     lens2[96:98] = 700
-    for lens_case, staged16 in ((lens, True), (lens2, False)):
+    # and rows of at most 27 entries: SELL tiles run
+    lens3 = rng.integers(0, 28, n)
+    for lens_case in (lens, lens2, lens3):
         Ao = ragged_matrix(oracle_mod, rng, lens_case, n)
         A = mfe.CSRMatrix.from_host(Ao.row_offsets, Ao.cols, Ao.vals)
         for nrhs in (4, 8, 16):
@@ -81,12 +102,8 @@ def test_spmm_ragged_rows(mfe, oracle_mod):
             tun = mfe.spmm_tune(A, nrhs)
             say(f"ragged nrhs={nrhs} max len {lens_case.max()}: ms {np.round(tun['ms'], 4).tolist()}")
             for tile, (ms, shape) in enumerate(zip(tun["ms"], tun["tiles"])):
-                if not shape["staged"]:
-                    assert np.isfinite(ms), (tile, shape)
-                elif shape["rows_per_block"] == 32 or not staged16:
-                    assert not np.isfinite(ms), (tile, shape)
-                elif nrhs <= 8:  # 16 rows x 39 entries of 8 doubles fit in LDS
-                    assert np.isfinite(ms), (tile, shape)
+                want_ok = tile_expected_available(shape, Ao.row_offsets, Ao.cols, n, nrhs)
+                assert bool(np.isfinite(ms)) == want_ok, (tile, shape, ms)
                 if not np.isfinite(ms):
                     with pytest.raises(RuntimeError):
                         mfe.spmm(A, Xd, Yd, nrhs, tile)
@@ -129,8 +146,8 @@ def test_multi_cg_matches_oracle(mfe, oracle_mod):
             assert np.max(np.abs(Xh[:, k] - ref["x"])) <= 1e-8 * np.max(np.abs(ref["x"]))
 
 
-def test_multi_cg_staged_tiles(mfe, oracle_mod):
-    """Each staged tile pinned in turn: the CG still matches the oracle's."""
+def test_multi_cg_pinned_tiles(mfe, oracle_mod):
+    """Each SELL and staged tile pinned in turn: the CG still matches the oracle's."""
     dims = (13, 9, 10)
     P = oracle_mod.build_problem(*dims)
     Ao = P["A"]
@@ -145,9 +162,9 @@ def test_multi_cg_staged_tiles(mfe, oracle_mod):
             refs.append(oracle_mod.cg_solve(Ao, bk, max_iter=3000,
                                             tol=1e-10 * float(np.linalg.norm(bk))))
         for tile, shape in enumerate(mfe.spmm_tiles(nrhs)):
-            if not shape["staged"]:
+            if shape["kind"] == "csr":
                 continue
-            say(f"multi cg staged {dims} nrhs={nrhs} tile {tile} {shape}")
+            say(f"multi cg pinned {dims} nrhs={nrhs} tile {tile} {shape}")
             mfe.spmm_select(A, nrhs, tile)
             X = mfe.Vector(n * nrhs)
             r = mfe.cg_solve_multi(A, mfe.Vector.from_numpy(B.ravel()), X, nrhs, max_iter=3000,
