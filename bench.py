@@ -49,14 +49,17 @@ def sell_bytes(n: int, plan: dict, value_bytes: int = 8) -> int:
 
 
 def sellx_bytes(n: int, plan: dict, value_bytes: int = 8) -> int:
-    """HBM bytes of one SELL-64 SpMV with staged x: every stored entry as value
-    + 16-bit local column, one length byte per row slot, the slice offsets,
-    the plan's runs (first column, list position) and per-tile run offsets and
-    distinct counts, x read once and y written once."""
+    """HBM bytes of one SELL-64 SpMV with staged x: every stored entry's value,
+    a 16-bit local column for the entries of slices without bases, one flag
+    byte per slice and 27 16-bit bases per slice with bases, one length byte
+    per row slot, the slice offsets, the plan's runs (first column, list
+    position) and per-tile run offsets and distinct counts, x read once and y
+    written once."""
     slots, ns, runs = plan["stored"], plan["slices"], plan["runs"]
     tiles = -(-n // 256)
-    return ((value_bytes + 2) * slots + 64 * ns + 4 * (ns + 1) + 8 * runs + 4 * (tiles + 1)
-            + 4 * tiles + 2 * value_bytes * n)
+    return (value_bytes * slots + 2 * plan["explicit_entries"] + ns + 54 * plan["base_slices"]
+            + 64 * ns + 4 * (ns + 1) + 8 * runs + 4 * (tiles + 1) + 4 * tiles
+            + 2 * value_bytes * n)
 
 
 def plan_bytes(n: int, nnz: int, plan: dict, value_bytes: int = 8) -> int:

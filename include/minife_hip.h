@@ -140,8 +140,9 @@ int mfe_make_local_matrix(mfe_ctx* ctx, mfe_csr* A);
  * 64 consecutive values/columns -- built on first use and rebuilt after the
  * values or structure change -- and, per 256-row tile, a staged-x plan: the
  * tile's distinct columns as runs of consecutive columns, with 16-bit local
- * columns per slot, so the SpMV copies the tile's x entries into LDS once and
- * gathers from there.  A tile with more than 128 runs leaves the matrix on
+ * columns per slot (or, for a slice whose local columns are base_j + lane,
+ * just the 27 bases), so the SpMV copies the tile's x entries into LDS once
+ * and gathers from there.  A tile with more than 128 runs leaves the matrix on
  * plain SELL-64; a row longer than 27 entries, or a copy that does not fit in
  * device memory, on the CSR kernels.  MFE_SPMV_SELL: SELL-64 without the
  * staged x.  MFE_SPMV_CSR frees the copies and uses the CSR kernels only.
@@ -153,9 +154,11 @@ int mfe_make_local_matrix(mfe_ctx* ctx, mfe_csr* A);
 /* Sets A's format and builds its FP64 plan now (so later solves do not pay
  * for it).  info (nullable) receives {kernel in use: 0 CSR row-stream,
  * 1 SELL-64, 2 CSR row-per-thread, 3 SELL-64 with staged x; stored entries
- * incl. padding; slices; longest row; runs of the staged-x plan (0 if not in
- * use); most distinct columns of a 256-row tile (0 if not in use)}. */
-int mfe_spmv_plan(mfe_ctx* ctx, mfe_csr* A, int format, int64_t info[6]);
+ * incl. padding; slices; longest row; and for the staged-x plan (else 0):
+ * runs; most distinct columns of a 256-row tile; stored entries whose local
+ * columns are read; slices read as base + lane instead (every row of the
+ * slice as wide as the slice, slot j's local column = base_j + lane)}. */
+int mfe_spmv_plan(mfe_ctx* ctx, mfe_csr* A, int format, int64_t info[8]);
 /* matvec_std<CSRMatrix>::operator()(A, x, y), SparseMatrix_functions.hpp:545-578.
  * Each row's products are summed in stored column order: bit-identical. */
 int mfe_matvec(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* x, mfe_vec* y);

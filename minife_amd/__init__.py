@@ -466,10 +466,11 @@ def spmv_plan(A: CSRMatrix, fmt: int = SPMV_AUTO) -> dict:
     every row has at most 27 entries, with x staged in LDS per 256-row tile
     when every tile's distinct columns form at most 128 runs; SPMV_SELL:
     SELL-64 without the staging; SPMV_CSR: CSR kernels only."""
-    info = (ctypes.c_int64 * 6)()
+    info = (ctypes.c_int64 * 8)()
     A.ctx.check(lib().mfe_spmv_plan(A.ctx.h, A.h, int(fmt), info), "spmv_plan")
     return dict(kernel=SPMV_KERNELS[info[0]], stored=int(info[1]), slices=int(info[2]),
-                longest_row=int(info[3]), runs=int(info[4]), max_distinct=int(info[5]))
+                longest_row=int(info[3]), runs=int(info[4]), max_distinct=int(info[5]),
+                explicit_entries=int(info[6]), base_slices=int(info[7]))
 
 
 def matvec(A: CSRMatrix, x: Vector, y: Vector) -> None:

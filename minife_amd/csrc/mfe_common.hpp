@@ -21,6 +21,7 @@ constexpr int kNumXcd = 8;         // MI355X: 8 XCDs, one L2 each
 constexpr int kSpmvRows = 256;     // rows per SpMV workgroup (one per thread)
 constexpr int kSpmvMaxNnz = 27 * kSpmvRows;  // LDS product slots per workgroup
 constexpr int kVecBlocks = 2048;   // fixed grid of the vector kernels
+constexpr int kSellBaseStride = 32;  // per-slice local-column bases (27 used)
 constexpr int kSellMaxWidth = 27;  // longest row the SELL SpMV takes (27-point stencil)
 
 struct CgWorkspace;     // mfe_solve.hip
@@ -85,6 +86,13 @@ struct mfe_csr {
   int* sellx_rcol = nullptr;              // runs: first column of the run
   int* sellx_rpos = nullptr;              // runs: its position in the tile's list
   int* sellx_nu = nullptr;                // ntiles: distinct columns of the tile
+  // Slices whose 64 rows all have the slice's width and whose slot-j local
+  // columns are base_j + lane (rows of one grid line on a stencil): the SpMV
+  // reads the 27 bases instead of 64 x 27 local columns.
+  int64_t sellx_affine = 0;                // such slices
+  int64_t sellx_explicit = 0;              // stored entries of the other slices
+  unsigned char* sellx_aff = nullptr;      // nslices: 1 = bases apply
+  unsigned short* sellx_sbase = nullptr;   // nslices x kSellBaseStride bases (0 past the width)
 };
 
 struct mfe_vec {

@@ -103,22 +103,25 @@ struct SellLocalRaw {
   typename SellVec2<T>::type vp[kSellMaxWidth / 2];
 };
 
+// Values of the slice at o64, local columns of the slice at lo64 of lcols (the
+// same position, or a zero region for slices whose columns are not read).
 template <typename T>
-__device__ __forceinline__ void sell_load_local(const unsigned short* lcols, const T* vals,
-                                                int64_t o64, int w, int lane, SellLocalRaw<T>& r) {
+__device__ __forceinline__ void sell_load_local(const unsigned short* lcols, int64_t lo64,
+                                                const T* vals, int64_t o64, int w, int lane,
+                                                SellLocalRaw<T>& r) {
   using V2 = typename SellVec2<T>::type;
   constexpr int kPairs = kSellMaxWidth / 2;  // 13: slots 1..26 (odd) or 0..25 (even)
   r.odd = w & 1;
   const int np = (w - r.odd) >> 1;
   const int last = np > 0 ? np - 1 : 0;
-  const int64_t pb = o64 + (r.odd ? 64 : 0);
-  r.c0 = lcols[o64 + lane];
+  const int first = r.odd ? 64 : 0;
+  r.c0 = lcols[lo64 + lane];
   r.v0 = vals[o64 + lane];
 #pragma unroll
   for (int p = 0; p < kPairs; ++p) {
-    const int64_t at = pb + (int64_t)(p < last ? p : last) * 128 + lane * 2;
-    r.cp[p] = *reinterpret_cast<const sell_us2*>(lcols + at);
-    r.vp[p] = *reinterpret_cast<const V2*>(vals + at);
+    const int64_t d = first + (int64_t)(p < last ? p : last) * 128 + lane * 2;
+    r.cp[p] = *reinterpret_cast<const sell_us2*>(lcols + lo64 + d);
+    r.vp[p] = *reinterpret_cast<const V2*>(vals + o64 + d);
   }
 }
 

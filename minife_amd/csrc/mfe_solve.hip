@@ -229,6 +229,8 @@ constexpr int kXMaxRuns = 128;     // runs per tile the SpMV keeps in LDS
 constexpr int kXLdsBytes = 56 * 1024;  // staged x per tile (27 * 256 doubles fit)
 constexpr int kXFirst = 10;  // x entries per thread requested before the slot loads
                              // (2560 per tile: a 27-point tile needs 9 * 258)
+constexpr int kXTail = 2048;  // zero local columns after the last slot: the clamped
+                              // loads' tail, and what slices with bases load instead
 
 // Exclusive workgroup scan of two counts; totals in ta, tb.
 __device__ __forceinline__ int2 block_scan2(int a, int b, int* sa, int* sb, int& ta, int& tb) {
@@ -251,13 +253,16 @@ __device__ __forceinline__ int2 block_scan2(int a, int b, int* sa, int* sb, int&
 // One workgroup per tile: the tile's columns sorted in LDS (bitonic); a value
 // is distinct where it differs from its predecessor and starts a run where the
 // predecessor is not value - 1.  Pass 0: counts[tile] = {distinct, runs}, or
-// {-1, -1} past kXCap non-zeros.  Pass 1: the runs at roff[tile], and the
-// local columns at every slot of the tile's slices (padding slots: 0).
+// {-1, -1} past kXCap non-zeros.  Pass 1: the runs at roff[tile], the local
+// columns at every slot of the tile's slices (padding slots: 0), and per slice
+// (one wave) whether every row has the slice's width w > 0 and slot j's local
+// column is base_j + lane for every j (saff = 1, bases in sbase; 0 past w).
 __global__ __launch_bounds__(kBlock) void sellx_plan_kernel(
     int n, int nslices, const int* __restrict__ ro, const int* __restrict__ cols,
     const int* __restrict__ soff, const unsigned char* __restrict__ slen, int pass,
     int2* __restrict__ counts, const int* __restrict__ roff, int* __restrict__ rcol,
-    int* __restrict__ rpos, unsigned short* __restrict__ lcols) {
+    int* __restrict__ rpos, unsigned short* __restrict__ lcols, unsigned char* __restrict__ saff,
+    unsigned short* __restrict__ sbase) {
   __shared__ int a[kXCap];
   __shared__ int sa[kBlock], sb[kBlock];
   constexpr int kPer = kXCap / kBlock;
@@ -323,10 +328,12 @@ __global__ __launch_bounds__(kBlock) void sellx_plan_kernel(
   const int slice = tile * kSlicesPerBlock + t / kSlice;
   if (slice >= nslices) return;
   const int row = tile * kSpmvRows + t;
+  const int lane = t % kSlice;
   const int o = soff[slice], w = soff[slice + 1] - o;
   const int len = slen[row];
   const int start = row < n ? ro[row] : 0;
-  for (int j = 0; j < w; ++j) {
+  bool affine = w > 0 && len == w;
+  for (int j = 0; j < w; ++j) {  // w is uniform over the wave
     int loc = 0;
     if (j < len) {
       const int c = cols[start + j];
@@ -338,12 +345,20 @@ __global__ __launch_bounds__(kBlock) void sellx_plan_kernel(
       }
       loc = lo;
     }
-    lcols[sell_pos((int64_t)o * kSlice, j, w, t % kSlice)] = (unsigned short)loc;
+    lcols[sell_pos((int64_t)o * kSlice, j, w, lane)] = (unsigned short)loc;
+    const int base = __shfl(loc, 0, kWave);
+    affine = affine && loc == base + lane;
+    if (lane == 0) sbase[(int64_t)slice * kSellBaseStride + j] = (unsigned short)base;
   }
+  if (lane >= w && lane < kSellBaseStride) sbase[(int64_t)slice * kSellBaseStride + lane] = 0;
+  const int all = __all(affine ? 1 : 0);
+  if (lane == 0) saff[slice] = (unsigned char)(all ? 1 : 0);
 }
 
 // spmv_sell_kernel's modes and results, x staged per tile (dynamic LDS:
-// A->sellx_maxu entries of T).  Order of work: the tile's run list goes to
+// max(A->sellx_maxu, 64) entries of T).  A slice with bases (saff) reads its
+// local columns as base_j + lane; its column loads go to the zero tail ltail
+// (one cached region) so that every wave issues the same loads.  Order of work: the tile's run list goes to
 // LDS; the thread's first kXFirst x entries are requested, then the lane's
 // slot loads (memory returns loads in issue order, so the x entries can be
 // written to LDS while the slot loads are still in flight); remaining x
@@ -353,10 +368,12 @@ __global__ __launch_bounds__(kBlock) void sellx_plan_kernel(
 template <typename T, bool ZERO_Y, int MODE>
 __global__ __launch_bounds__(kBlock) void spmv_sellx_kernel(
     int n, int nslices, const int* __restrict__ soff, const unsigned char* __restrict__ slen,
-    const unsigned short* __restrict__ lcols, const T* __restrict__ vals,
-    const int* __restrict__ roff, const int* __restrict__ rcol, const int* __restrict__ rpos,
-    const int* __restrict__ tnu, const T* __restrict__ x, T* __restrict__ y,
-    const T* __restrict__ bvec, T* __restrict__ partials, const int* __restrict__ done) {
+    const unsigned short* __restrict__ lcols, const unsigned short* __restrict__ ltail,
+    const unsigned char* __restrict__ saff, const unsigned short* __restrict__ sbase,
+    const T* __restrict__ vals, const int* __restrict__ roff, const int* __restrict__ rcol,
+    const int* __restrict__ rpos, const int* __restrict__ tnu, const T* __restrict__ x,
+    T* __restrict__ y, const T* __restrict__ bvec, T* __restrict__ partials,
+    const int* __restrict__ done) {
   if (done && *done) return;
   constexpr int CH = sizeof(T) == 8 ? 8 : 4;  // x loads in flight per thread while staging
   extern __shared__ __align__(16) unsigned char sx_bytes[];
@@ -381,6 +398,14 @@ __global__ __launch_bounds__(kBlock) void spmv_sellx_kernel(
   const int o = soff[sl];
   const int w = live ? soff[sl + 1] - o : 0;
   const int len = live ? (int)slen[row] : 0;
+  // per-wave values: the slice's flag and its 32 bases (64 bytes) are read
+  // before the barrier, from a uniform address
+  const int sl_u = __builtin_amdgcn_readfirstlane(sl);
+  const bool aff = live && saff[sl_u];
+  const uint4* sb4 = reinterpret_cast<const uint4*>(sbase + (int64_t)sl_u * kSellBaseStride);
+  const uint4 q0 = sb4[0], q1 = sb4[1], q2 = sb4[2], q3 = sb4[3];
+  const unsigned bw[16] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                           q2.x, q2.y, q2.z, q2.w, q3.x, q3.y, q3.z, q3.w};
   if (has_run) {
     run_col[t] = my_col;
     run_pos[t] = my_pos;
@@ -401,7 +426,8 @@ __global__ __launch_bounds__(kBlock) void spmv_sellx_kernel(
     for (int k = 0; k < kXFirst; ++k) xf[k] = x[g[k]];
   }
   SellLocalRaw<T> raw;
-  sell_load_local<T>(lcols, vals, (int64_t)o * kSlice, w, lane, raw);
+  sell_load_local<T>(aff ? ltail : lcols, aff ? 0 : (int64_t)o * kSlice, vals,
+                     (int64_t)o * kSlice, w, lane, raw);
 #pragma unroll
   for (int k = 0; k < kXFirst; ++k)
     if (t + k * kBlock < nu) sx[t + k * kBlock] = xf[k];
@@ -427,7 +453,10 @@ __global__ __launch_bounds__(kBlock) void spmv_sellx_kernel(
     int c[kSellMaxWidth];
     T v[kSellMaxWidth], xv[kSellMaxWidth];
 #pragma unroll
-    for (int j = 0; j < kSellMaxWidth; ++j) sell_slot<T>(raw, j, c[j], v[j]);
+    for (int j = 0; j < kSellMaxWidth; ++j) {
+      sell_slot<T>(raw, j, c[j], v[j]);
+      if (aff) c[j] = (int)((bw[j >> 1] >> ((j & 1) * 16)) & 0xffffu) + lane;
+    }
 #pragma unroll
     for (int j = 0; j < kSellMaxWidth; ++j) xv[j] = sx[c[j]];
 #pragma unroll
@@ -449,11 +478,15 @@ static void sellx_release(mfe_csr* A) {
   (void)hipFree(A->sellx_rcol);
   (void)hipFree(A->sellx_rpos);
   (void)hipFree(A->sellx_nu);
+  (void)hipFree(A->sellx_aff);
+  (void)hipFree(A->sellx_sbase);
   A->sellx_lcols = nullptr;
   A->sellx_roff = A->sellx_rcol = A->sellx_rpos = A->sellx_nu = nullptr;
+  A->sellx_aff = nullptr;
+  A->sellx_sbase = nullptr;
   A->sellx_valid = A->sellx_ok = false;
   A->sellx_maxu = 0;
-  A->sellx_ntiles = A->sellx_runs = 0;
+  A->sellx_ntiles = A->sellx_runs = A->sellx_affine = A->sellx_explicit = 0;
 }
 
 void sell_release(mfe_csr* A) {
@@ -509,7 +542,7 @@ static int build_sellx(mfe_ctx* ctx, mfe_csr* A) {
   if (!try_alloc(&d_counts, sizeof(int2) * (size_t)nt)) return MFE_OK;
   sellx_plan_kernel<<<nt, kBlock, 0, s>>>((int)n, ns, A->row_offsets, A->cols, A->sell_off,
                                            A->sell_len, 0, d_counts, nullptr, nullptr, nullptr,
-                                           nullptr);
+                                           nullptr, nullptr, nullptr);
   std::vector<int2> counts((size_t)nt);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess)
@@ -530,28 +563,44 @@ static int build_sellx(mfe_ctx* ctx, mfe_csr* A) {
   }
   roff[(size_t)nt] = (int)runs;
   if ((int64_t)maxu * (int64_t)sizeof(double) > kXLdsBytes || runs >= INT32_MAX) return MFE_OK;
-  const size_t nl = (size_t)(A->sell_slots + kSellTail);
+  const size_t nl = (size_t)(A->sell_slots + kXTail);
   const bool ok = try_alloc(&A->sellx_lcols, sizeof(unsigned short) * nl) &&
                   try_alloc(&A->sellx_roff, sizeof(int) * ((size_t)nt + 1)) &&
                   try_alloc(&A->sellx_rcol, sizeof(int) * (size_t)std::max<int64_t>(runs, 1)) &&
                   try_alloc(&A->sellx_rpos, sizeof(int) * (size_t)std::max<int64_t>(runs, 1)) &&
-                  try_alloc(&A->sellx_nu, sizeof(int) * (size_t)nt);
+                  try_alloc(&A->sellx_nu, sizeof(int) * (size_t)nt) &&
+                  try_alloc(&A->sellx_aff, (size_t)ns) &&
+                  try_alloc(&A->sellx_sbase, sizeof(unsigned short) * kSellBaseStride * (size_t)ns);
   if (!ok) {
     sellx_release(A);
     A->sellx_valid = true;
     return MFE_OK;
   }
   MFE_CHECK(ctx, hipMemsetAsync(A->sellx_lcols + A->sell_slots, 0,
-                                sizeof(unsigned short) * kSellTail, s));
+                                sizeof(unsigned short) * kXTail, s));
   MFE_CHECK(ctx, hipMemcpyAsync(A->sellx_roff, roff.data(), sizeof(int) * roff.size(),
                                 hipMemcpyHostToDevice, s));
   MFE_CHECK(ctx, hipMemcpyAsync(A->sellx_nu, nu.data(), sizeof(int) * nu.size(),
                                 hipMemcpyHostToDevice, s));
   sellx_plan_kernel<<<nt, kBlock, 0, s>>>((int)n, ns, A->row_offsets, A->cols, A->sell_off,
                                            A->sell_len, 1, nullptr, A->sellx_roff, A->sellx_rcol,
-                                           A->sellx_rpos, A->sellx_lcols);
+                                           A->sellx_rpos, A->sellx_lcols, A->sellx_aff,
+                                           A->sellx_sbase);
   MFE_LAUNCHED(ctx, "sellx_plan_kernel fill");
+  // slices with bases, and the stored entries of the others (plan figures)
+  std::vector<unsigned char> aff((size_t)ns);
+  std::vector<int> soff((size_t)ns + 1);
+  MFE_CHECK(ctx, hipMemcpyAsync(aff.data(), A->sellx_aff, (size_t)ns, hipMemcpyDeviceToHost, s));
+  MFE_CHECK(ctx, hipMemcpyAsync(soff.data(), A->sell_off, sizeof(int) * soff.size(),
+                                hipMemcpyDeviceToHost, s));
   MFE_CHECK(ctx, hipStreamSynchronize(s));  // roff/nu host copies leave scope
+  int64_t naff = 0, nexp = 0;
+  for (int i = 0; i < ns; ++i) {
+    if (aff[(size_t)i]) ++naff;
+    else nexp += (int64_t)(soff[(size_t)i + 1] - soff[(size_t)i]) * kSlice;
+  }
+  A->sellx_affine = naff;
+  A->sellx_explicit = nexp;
   A->sellx_ok = true;
   A->sellx_maxu = maxu;
   A->sellx_ntiles = nt;
@@ -688,6 +737,9 @@ struct SpmvOp {
   const int* scols = nullptr;
   const T* svals = nullptr;  // non-null: SELL
   const unsigned short* xl = nullptr;  // non-null: SELL with staged x
+  const unsigned short* xltail = nullptr;
+  const unsigned char* xaff = nullptr;
+  const unsigned short* xsbase = nullptr;
   const int *xroff = nullptr, *xrcol = nullptr, *xrpos = nullptr, *xnu = nullptr;
   int xmaxu = 0;
   const void* key() const { return svals ? (const void*)svals : (const void*)vals; }
@@ -720,6 +772,9 @@ static SpmvOp<T> spmv_op(const mfe_csr* A, bool allow_sell = true, bool allow_st
     op.scols = A->sell_cols;
     if (allow_stage && A->spmv_format == MFE_SPMV_AUTO && A->sellx_ok) {
       op.xl = A->sellx_lcols;
+      op.xltail = A->sellx_lcols + A->sell_slots;
+      op.xaff = A->sellx_aff;
+      op.xsbase = A->sellx_sbase;
       op.xroff = A->sellx_roff;
       op.xrcol = A->sellx_rcol;
       op.xrpos = A->sellx_rpos;
@@ -742,10 +797,11 @@ static int launch_spmv(const SpmvOp<T>& op, const T* x, T* y, const T* bvec, T* 
   }
   const int tiles = (int)cdiv(n, kSpmvRows);
   if (op.xl) {
-    const size_t lds = sizeof(T) * (size_t)std::max(op.xmaxu, 1);  // padding slots read entry 0
+    // padding slots read entry 0, slots past the width of a slice with bases entry lane
+    const size_t lds = sizeof(T) * (size_t)std::max(op.xmaxu, kSlice);
     spmv_sellx_kernel<T, ZERO_Y, MODE><<<tiles, kBlock, lds, stream>>>(
-        n, op.nslices, op.soff, op.slen, op.xl, op.svals, op.xroff, op.xrcol, op.xrpos, op.xnu, x,
-        y, bvec, partials, done);
+        n, op.nslices, op.soff, op.slen, op.xl, op.xltail, op.xaff, op.xsbase, op.svals, op.xroff,
+        op.xrcol, op.xrpos, op.xnu, x, y, bvec, partials, done);
   } else if (op.svals) {
     spmv_sell_kernel<T, ZERO_Y, MODE><<<tiles, kBlock, 0, stream>>>(
         n, op.nslices, op.soff, op.slen, op.scols, op.svals, x, y, bvec, partials, done);
@@ -1574,7 +1630,7 @@ static int read_scalar(mfe_ctx* ctx, const double* d, double* out) {
   return MFE_OK;
 }
 
-int mfe_spmv_plan(mfe_ctx* ctx, mfe_csr* A, int format, int64_t info[6]) {
+int mfe_spmv_plan(mfe_ctx* ctx, mfe_csr* A, int format, int64_t info[8]) {
   if (!ctx || !A || (format != MFE_SPMV_AUTO && format != MFE_SPMV_CSR && format != MFE_SPMV_SELL))
     return MFE_BAD_ARG;
   MFE_CHECK(ctx, hipSetDevice(ctx->device));
@@ -1602,6 +1658,8 @@ int mfe_spmv_plan(mfe_ctx* ctx, mfe_csr* A, int format, int64_t info[6]) {
     info[3] = longest;
     info[4] = op.xl ? A->sellx_runs : 0;
     info[5] = op.xl ? A->sellx_maxu : 0;
+    info[6] = op.xl ? A->sellx_explicit : 0;
+    info[7] = op.xl ? A->sellx_affine : 0;
   }
   MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return MFE_OK;

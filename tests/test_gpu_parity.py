@@ -233,6 +233,32 @@ def staged_plan_model(ro, cols, n, rows=256):
     return total, maxr, maxu
 
 
+def base_slices_model(ro, cols, n):
+    """(stored entries of slices read with explicit local columns, slices read
+    as base + lane): a 64-row slice qualifies when all 64 rows (rows past n
+    count as empty) have the slice's width w > 0 and, for every slot j < w,
+    the position of row r's j-th column in its 256-row tile's sorted distinct
+    columns is that of the slice's first row plus r - first row."""
+    ns = -(-n // 64)
+    lens = np.zeros(ns * 64, np.int64)
+    lens[:n] = np.diff(ro)
+    explicit, base = 0, 0
+    for sl in range(ns):
+        r0 = sl * 64
+        w = int(lens[r0:r0 + 64].max())
+        ok = w > 0 and bool(np.all(lens[r0:r0 + 64] == w))
+        if ok:
+            t0 = (r0 // 256) * 256
+            u = np.unique(cols[ro[t0]:ro[min(t0 + 256, n)]])
+            loc = np.stack([np.searchsorted(u, cols[ro[r]:ro[r] + w]) for r in range(r0, r0 + 64)])
+            ok = bool(np.all(loc == loc[0][None, :] + np.arange(64)[:, None]))
+        if ok:
+            base += 1
+        else:
+            explicit += w * 64
+    return explicit, base
+
+
 def expected_auto_kernel(ro, cols, n):
     if n == 0 or int(np.diff(ro).max()) > 27:
         return None
@@ -277,6 +303,7 @@ def test_sell_and_csr_kernels_bit_exact_on_ragged_rows(mfe, ctx, oracle_mod, n):
         if kernel == "sell64-staged-x":
             runs, _, maxu = staged_plan_model(ro, cols, n)
             assert (info["runs"], info["max_distinct"]) == (runs, maxu)
+            assert (info["explicit_entries"], info["base_slices"]) == base_slices_model(ro, cols, n)
         x = mfe.Vector.from_numpy(xh, ctx)
         y = mfe.Vector.from_numpy(yh, ctx)
         mfe.matvec(A, x, y)
@@ -357,6 +384,44 @@ def test_cg_identical_across_spmv_formats(mfe, ctx):
         assert bits_equal(out[0][1], other[1])
 
 
+@pytest.mark.parametrize("dims", [(199, 4, 3), (130, 2, 2)])
+def test_staged_x_base_slices_on_long_lines(mfe, ctx, oracle_mod, dims):
+    """Meshes whose grid lines hold whole 64-row slices: those slices are read
+    as base + lane (counted against a numpy model); SpMV, SpMV+dot, FP32 SpMV
+    and the residual form equal the oracle bit for bit, and the CG is
+    bit-identical to the plain SELL-64 format's."""
+    say(f"case base slices {dims}")
+    P = oracle_mod.build_problem(*dims)
+    Ao = P["A"]
+    n = Ao.nrows
+    out = []
+    for fmt, kernel in [(mfe.SPMV_AUTO, "sell64-staged-x"), (mfe.SPMV_SELL, "sell64")]:
+        A = mfe.CSRMatrix.from_host(Ao.row_offsets, Ao.cols, Ao.vals, ctx=ctx)
+        mfe.make_local_matrix(A)
+        info = mfe.spmv_plan(A, fmt)
+        assert info["kernel"] == kernel
+        if kernel == "sell64-staged-x":
+            model = base_slices_model(Ao.row_offsets, Ao.cols, n)
+            assert model[1] > 0
+            assert (info["explicit_entries"], info["base_slices"]) == model
+        xh = np.random.default_rng(5).standard_normal(n)
+        x = mfe.Vector.from_numpy(xh, ctx)
+        y = mfe.Vector(n, ctx)
+        mfe.matvec(A, x, y)
+        want = oracle_mod.matvec(Ao, xh)
+        assert bits_equal(y.to_numpy(), want), kernel
+        mfe.matvec_and_dot(A, x, y)
+        assert bits_equal(y.to_numpy(), want), kernel
+        mfe.matvec_fp32(A, x, y)
+        assert bits_equal(y.to_numpy(), oracle_mod.matvec_f32(Ao, xh)), kernel
+        b = mfe.Vector.from_numpy(P["b"], ctx)
+        xs = mfe.Vector(n, ctx)
+        r = mfe.cg_solve(A, b, xs, max_iter=80, tolerance=0.0, history=True)
+        out.append((xs.to_numpy(), r.history))
+    assert bits_equal(out[0][0], out[1][0])
+    assert bits_equal(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("band,kernel", [(300, "sell64-staged-x"), (None, "sell64")])
 def test_staged_x_on_banded_and_scattered_columns(mfe, ctx, oracle_mod, band, kernel):
     """20,000 rows of 1..27 random columns: within +-300 of the row (irregular
@@ -382,6 +447,7 @@ def test_staged_x_on_banded_and_scattered_columns(mfe, ctx, oracle_mod, band, ke
     if kernel == "sell64-staged-x":
         runs, _, maxu = staged_plan_model(ro, cols, n)
         assert (info["runs"], info["max_distinct"]) == (runs, maxu)
+        assert (info["explicit_entries"], info["base_slices"]) == base_slices_model(ro, cols, n)
     xh = rng.standard_normal(n)
     x = mfe.Vector.from_numpy(xh, ctx)
     y = mfe.Vector(n, ctx)
