@@ -225,18 +225,45 @@ __device__ void pad_row(const BatchProb& P, int id, const int* soff, int* scols,
   }
 }
 
-// (A p)[row] for row < n of problem P from its SELL slices, summed over the
-// row's entries in stored column order (the row length is the box stencil's).
+// (A p)[row] for row < n of problem P from its SELL values, summed over the
+// row's entries in stored (ascending column) order, 0 for padding rows.  The
+// columns are not loaded: a box row's k-th stored entry is its k-th in-range
+// neighbour in (z, y, x) order, so each neighbour's x address and slot follow
+// from the row's grid position, and every value load and x gather of the row
+// is issued at once (out-of-range neighbours load slot 0 and p[row], unused).
 __device__ __forceinline__ double batch_row_sum(const BatchProb& P, int row, const int* soff,
-                                                const int* scols, const double* svals,
-                                                const double* pv) {
+                                                const double* svals, const double* pv) {
   const int slice = row / kWave, lane = row % kWave;
   const int o = soff[slice], w = soff[slice + 1] - o;
   const int gx = P.Nx + 1, gy = P.Ny + 1;
-  const int rr = row < P.n ? row : P.n - 1;
+  const bool valid = row < P.n;
+  const int rr = valid ? row : P.n - 1;
   const int ix = rr % gx, iy = (rr / gx) % gy, iz = rr / (gx * gy);
-  const int len = row < P.n ? axis_len(ix, gx) * axis_len(iy, gy) * axis_len(iz, P.Nz + 1) : 0;
-  return sell_row_sum<double>(scols, svals, pv, P.slot0 + (int64_t)o * kWave, w, lane, len, 0.0);
+  const int lox = ix > 0 ? -1 : 0, hix = ix < P.Nx ? 1 : 0;
+  const int loy = iy > 0 ? -1 : 0, hiy = iy < P.Ny ? 1 : 0;
+  const int loz = iz > 0 ? -1 : 0, hiz = iz < P.Nz ? 1 : 0;
+  const int64_t o64 = P.slot0 + (int64_t)o * kWave;
+  double v[27], xv[27];
+  bool in[27];
+  int pos = 0;
+#pragma unroll
+  for (int sz = -1; sz <= 1; ++sz)
+#pragma unroll
+    for (int sy = -1; sy <= 1; ++sy)
+#pragma unroll
+      for (int sx = -1; sx <= 1; ++sx) {
+        const int q = (sz + 1) * 9 + (sy + 1) * 3 + (sx + 1);
+        in[q] = valid && sx >= lox && sx <= hix && sy >= loy && sy <= hiy && sz >= loz &&
+                sz <= hiz;
+        v[q] = svals[sell_pos(o64, in[q] ? pos : 0, w, lane)];
+        xv[q] = pv[in[q] ? rr + sx + gx * (sy + gy * sz) : rr];
+        pos += in[q] ? 1 : 0;
+      }
+  double sum = 0.0;
+#pragma unroll
+  for (int q = 0; q < 27; ++q)
+    if (in[q]) sum += v[q] * xv[q];
+  return sum;
 }
 
 // WAVES: the minimum waves per SIMD the register allocation must allow (the
@@ -358,7 +385,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(WAVES, 8
     for (int tile = i; tile < ntiles; tile += nb) {
       const int row = tile * kBlock + t;
       if (row / kWave < P.nslices) {
-        const double sum = batch_row_sum(P, row, soff, scols, svals, pp);
+        const double sum = batch_row_sum(P, row, soff, svals, pp);
         if (row < P.n) {
           App[row] = sum;
           acc += sum * pp[row];
