@@ -49,18 +49,21 @@ def _compile(src: str) -> str:
 
 DRIVER_SRC = os.path.join(HERE, "host", "miniFE_hip_main.cpp")
 DRIVER = os.path.join(HERE, "miniFE_hip.x")
+DRIVER_ELL = os.path.join(HERE, "miniFE_hip_ell.x")  # the reference's MINIFE_ELL_MATRIX build
 
 
 def build_driver() -> str:
-    """The miniFE.x-compatible driver (host C++ over the C-ABI)."""
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
-           "-I", os.path.join(HERE, "host"), DRIVER_SRC, "-o", DRIVER + ".tmp", "-L", HERE,
-           "-lminife_hip", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN",
-           "-Wl,-rpath,/opt/rocm/lib"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"driver build failed:\n{r.stdout}\n{r.stderr}")
-    os.replace(DRIVER + ".tmp", DRIVER)
+    """The miniFE.x-compatible driver (host C++ over the C-ABI), with the CSR
+    matrix type and, as miniFE_hip_ell.x, with the ELL one."""
+    for exe, defs in ((DRIVER, []), (DRIVER_ELL, ["-DMINIFE_ELL_MATRIX"])):
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", *defs, "-I", os.path.join(ROOT, "include"),
+               "-I", os.path.join(HERE, "host"), DRIVER_SRC, "-o", exe + ".tmp", "-L", HERE,
+               "-lminife_hip", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+               "-Wl,-rpath,/opt/rocm/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"driver build failed:\n{r.stdout}\n{r.stderr}")
+        os.replace(exe + ".tmp", exe)
     return DRIVER
 
 

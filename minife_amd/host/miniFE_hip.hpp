@@ -11,6 +11,7 @@
 //   Parameters                utils/Parameters.hpp     miniFE::Parameters
 //   simple_mesh_description   src/simple_mesh_description.hpp:38-244
 //   CSRMatrix<S,LO,GO>        src/CSRMatrix.hpp:40-129 (device-resident)
+//   ELLMatrix<S,LO,GO>        src/ELLMatrix.hpp:40-134 (CSR storage, ELL counts)
 //   Vector<S,LO,GO>           src/Vector.hpp:36-61     (device-resident)
 //   generate_matrix_structure src/generate_matrix_structure.hpp:47-149
 //   assemble_FE_data          src/assemble_FE_data.hpp:38-73
@@ -174,6 +175,33 @@ struct CSRMatrix {
   mfe_csr* h = nullptr;
 };
 
+// ELLMatrix.hpp:40-134: every row reserves num_cols_per_row slots (27, set by
+// generate_matrix_structure as reserve_space(nrows, 27) does); the stored
+// entries, their order and all arithmetic are the CSR matrix's -- the
+// reference's ELL and CSR builds write the same solution and the same matrix
+// entries -- and num_nonzeros() counts the reserved slots (rows x 27), which is
+// what the reference's ELL build reports as NNZ, in its memory and flop
+// figures and in the dump header.  On the device make_local_matrix plans the
+// sliced-ELL kernel (MFE_SPMV_SELL: 64-row slices as wide as their longest row).
+template <typename Scalar, typename LocalOrdinal, typename GlobalOrdinal>
+struct ELLMatrix : CSRMatrix<Scalar, LocalOrdinal, GlobalOrdinal> {
+  size_t num_nonzeros() const { return this->num_rows() * (size_t)num_cols_per_row; }
+  LocalOrdinal num_cols_per_row = 0;
+};
+
+// Per-type hooks of the operators below: the row capacity generate_matrix_structure
+// reserves, and the SpMV plan make_local_matrix builds.
+template <typename S, typename L, typename G>
+void reserve_row_capacity(CSRMatrix<S, L, G>&, int) {}
+template <typename S, typename L, typename G>
+void reserve_row_capacity(ELLMatrix<S, L, G>& A, int ncols_per_row) {
+  A.num_cols_per_row = ncols_per_row;
+}
+template <typename S, typename L, typename G>
+int spmv_format_for(const CSRMatrix<S, L, G>&) { return MFE_SPMV_AUTO; }
+template <typename S, typename L, typename G>
+int spmv_format_for(const ELLMatrix<S, L, G>&) { return MFE_SPMV_SELL; }
+
 template <typename Scalar, typename LocalOrdinal, typename GlobalOrdinal>
 struct Vector {
   typedef Scalar ScalarType;
@@ -211,6 +239,7 @@ int generate_matrix_structure(
                                                mesh.local_box.ranges, A.h);
   if (st == MFE_OOM) return 1;  // generate_matrix_structure.hpp:140-148
   check(st, "generate_matrix_structure");
+  reserve_row_capacity(A, 27);  // generate_matrix_structure.hpp:81
   A.has_local_indices = false;
   return 0;
 }
@@ -244,12 +273,13 @@ void impose_dirichlet(typename MatrixType::ScalarType prescribed_value, MatrixTy
 
 // The reference renumbers columns for its halo exchange here, the step
 // before the solve that prepares A for matvec; on one rank that renumbering is
-// the identity, and the SpMV storage plan (SELL-64 copy, staged-x runs) is
-// built here instead of inside the first cg_solve.
+// the identity, and the SpMV storage plan (SELL-64 copy, staged-x runs; the
+// plain sliced ELL for an ELLMatrix) is built here instead of inside the
+// first cg_solve.
 template <typename MatrixType>
 void make_local_matrix(MatrixType& A) {
   check(mfe_make_local_matrix(context(), A.h), "make_local_matrix");
-  check(mfe_spmv_plan(context(), A.h, MFE_SPMV_AUTO, nullptr), "make_local_matrix (SpMV plan)");
+  check(mfe_spmv_plan(context(), A.h, spmv_format_for(A), nullptr), "make_local_matrix (SpMV plan)");
   A.has_local_indices = true;
 }
 

@@ -102,7 +102,11 @@ T param(int argc, char** argv, const std::string& key, T dflt) {
 
 int main(int argc, char** argv) {
   using namespace miniFE;
+#if defined(MINIFE_ELL_MATRIX)  // driver.hpp:172-176
+  typedef ELLMatrix<double, int, int> MatrixType;
+#else
   typedef CSRMatrix<double, int, int> MatrixType;
+#endif
   typedef Vector<double, int, int> VectorType;
 
   Parameters params;

@@ -1,5 +1,6 @@
-"""GPU: the miniFE.x-compatible driver (minife_amd/miniFE_hip.x) end to end,
-through the C++ operator layer, against the reference's results."""
+"""GPU: the miniFE.x-compatible driver (minife_amd/miniFE_hip.x, and
+miniFE_hip_ell.x with the reference's ELL matrix type) end to end, through the
+C++ operator layer, against the reference's results."""
 import os
 import subprocess
 
@@ -11,11 +12,12 @@ from conftest import ROOT, golden_npz
 pytestmark = pytest.mark.gpu
 
 DRIVER = os.path.join(ROOT, "minife_amd", "miniFE_hip.x")
+DRIVER_ELL = os.path.join(ROOT, "minife_amd", "miniFE_hip_ell.x")
 
 
-def run_driver(tmp_path, *args):
-    print("driver", args, flush=True)
-    r = subprocess.run([DRIVER, *args], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+def run_driver(tmp_path, *args, exe=DRIVER):
+    print("driver", os.path.basename(exe), args, flush=True)
+    r = subprocess.run([exe, *args], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     yamls = [f for f in os.listdir(tmp_path) if f.endswith(".yaml")]
     assert len(yamls) == 1, (r.stdout, r.stderr)
     doc = {}
@@ -80,19 +82,26 @@ def test_driver_config_text_variant(tmp_path, oracle_mod):
 
 
 DUMPS = os.path.join(ROOT, "tests", "golden", "dumps")
+DUMPS_ELL = os.path.join(ROOT, "tests", "golden", "dumps_ell")
 
 
+@pytest.mark.parametrize("matrix", ["csr", "ell"])
 @pytest.mark.parametrize("dims", [(1, 1, 2), (2, 1, 1), (17, 3, 5), (20, 30, 40)])
-def test_driver_debug_dumps_match_reference(tmp_path, dims):
+def test_driver_debug_dumps_match_reference(tmp_path, dims, matrix):
     """The reference CI's parity check (diff of the MINIFE_DEBUG dumps between
     implementations, one rank): debug_dumps=1 replay=1 writes A_prebc, A,
     A_local, b_prebc, b, b_local and x files identical to the reference
-    miniFE.x's (gzipped fixtures; SHA-256 at 20x30x40)."""
+    miniFE.x's (gzipped fixtures; SHA-256 at 20x30x40) -- with the CSR matrix
+    type, and with the ELL one against the reference's ELL build (whose matrix
+    headers count 27 slots per row)."""
     import gzip
     import hashlib
     import json
+    DUMPS = DUMPS_ELL if matrix == "ell" else globals()["DUMPS"]
+    exe = DRIVER_ELL if matrix == "ell" else DRIVER
     nx, ny, nz = dims
-    r, _ = run_driver(tmp_path, f"nx={nx}", f"ny={ny}", f"nz={nz}", "debug_dumps=1", "replay=1")
+    r, _ = run_driver(tmp_path, f"nx={nx}", f"ny={ny}", f"nz={nz}", "debug_dumps=1", "replay=1",
+                      exe=exe)
     assert r.returncode == 0, r.stderr
     case = "x".join(map(str, dims))
     names = ["A_prebc.mtx.1.0", "A.mtx.1.0", "A_local.mtx.1.0", "b_prebc.vec.1.0", "b.vec.1.0",
@@ -105,6 +114,26 @@ def test_driver_debug_dumps_match_reference(tmp_path, dims):
         else:
             want = gzip.open(os.path.join(DUMPS, case, f + ".gz"), "rb").read()
             assert got == want, f
+
+
+def test_driver_ell_matrix_report(tmp_path):
+    """The ELL build's YAML matrix figures (NNZ = rows x 27 reserved slots, the
+    memory and flop figures that follow) and iteration count at 20x30x40 equal
+    the reference's ELL build's."""
+    import json
+    want = json.load(open(os.path.join(DUMPS_ELL, "20x30x40_yaml.json")))
+    r, doc = run_driver(tmp_path, "nx=20", "ny=30", "nz=40", "replay=1", exe=DRIVER_ELL)
+    assert r.returncode == 0, r.stderr
+    for key in ("Global NNZ", "NNZ per proc MIN", "NNZ per proc MAX"):
+        assert int(doc[f"Matrix attributes/{key}"]) == int(want[key]), key
+    assert float(doc["Matrix attributes/NNZ per proc AVG"]) == pytest.approx(
+        float(want["NNZ per proc AVG"]), rel=1e-6)
+    assert float(doc["Matrix attributes/Global Memory (GB)"]) == pytest.approx(
+        float(want["Global Memory (GB)"]), rel=1e-5)
+    assert int(doc["CG solve/Iterations"]) == int(want["Iterations"])
+    for path, key in (("CG solve/MATVEC Flops", "MATVEC Flops"),
+                      ("CG solve/Total/Total CG Flops", "Total CG Flops")):
+        assert float(doc[path]) == pytest.approx(float(want[key]), rel=1e-5), key
 
 
 def test_driver_time_kernels_mode(tmp_path, oracle_mod):
