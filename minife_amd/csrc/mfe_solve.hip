@@ -217,12 +217,14 @@ __global__ __launch_bounds__(kBlock) void sell_fill_kernel(
 // Workgroup = 4 slices = a tile of 256 rows.  A plan lists each tile's
 // distinct columns in increasing order, as runs of consecutive columns (first
 // column, position in the list), and stores at every SELL slot the column's
-// position in the tile's list as 16 bits.  The SpMV first issues the lane's
-// value and local-column loads (the HBM stream), then copies the tile's runs
-// of x into LDS (coalesced: a 256-row tile of the 27-point stencil has 9 runs,
-// 2.3 K entries), and after the barrier gathers x from LDS.  Index traffic
-// drops from 4 to 2 bytes per slot and x comes from L2 once per tile instead
-// of once per slot.  Sums are in stored column order: the other kernels' bits.
+// position in the tile's list as 16 bits; a slice whose slot-j positions are
+// base_j + lane (the rows of one grid line) keeps only its 27 bases.  The SpMV
+// copies the tile's runs of x into LDS (coalesced: a 256-row tile of the
+// 27-point stencil has 9 runs, 2.3 K entries) while the slot loads (the HBM
+// stream) are in flight, and after the barrier gathers x from LDS.  Index
+// traffic drops from 4 bytes per slot to 2, or to none for slices with bases,
+// and x comes from L2 once per tile instead of once per slot.  Sums are in
+// stored column order: the other kernels' bits.
 
 constexpr int kXCap = 8192;        // non-zeros per tile the plan kernel sorts in LDS
 constexpr int kXMaxRuns = 128;     // runs per tile the SpMV keeps in LDS
@@ -358,13 +360,14 @@ __global__ __launch_bounds__(kBlock) void sellx_plan_kernel(
 // spmv_sell_kernel's modes and results, x staged per tile (dynamic LDS:
 // max(A->sellx_maxu, 64) entries of T).  A slice with bases (saff) reads its
 // local columns as base_j + lane; its column loads go to the zero tail ltail
-// (one cached region) so that every wave issues the same loads.  Order of work: the tile's run list goes to
-// LDS; the thread's first kXFirst x entries are requested, then the lane's
-// slot loads (memory returns loads in issue order, so the x entries can be
-// written to LDS while the slot loads are still in flight); remaining x
-// entries, if any, follow; after the barrier the slots are unpacked and
-// summed.  Every local column in the array is below A->sellx_maxu, so the LDS
-// reads of padding and of clamped loads stay inside the allocation.
+// (one cached region) so that every wave issues the same loads.  Order of
+// work: the tile's run list goes to LDS; the thread's first kXFirst x entries
+// are requested, then the lane's slot loads (memory returns loads in issue
+// order, so the x entries can be written to LDS while the slot loads are
+// still in flight); remaining x entries, if any, follow; after the barrier
+// the slots are unpacked and summed.  Every local column in the array is
+// below A->sellx_maxu, and a base + lane below max(A->sellx_maxu, 64), so the
+// LDS reads of padding and of clamped loads stay inside the allocation.
 template <typename T, bool ZERO_Y, int MODE>
 __global__ __launch_bounds__(kBlock) void spmv_sellx_kernel(
     int n, int nslices, const int* __restrict__ soff, const unsigned char* __restrict__ slen,
