@@ -21,9 +21,11 @@ raises if ``libminife_hip.so`` is missing or no GPU is visible.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 import sys
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -94,11 +96,31 @@ def _share_torch_hip_runtime() -> None:
             return
 
 
+# Live handles: contexts, and what lives in a context (matrices, vectors,
+# batches).  At interpreter exit every one still open is closed, dependents
+# first, while the HIP runtime is up: none of the captured graphs, streams,
+# events or pinned buffers is left to the runtime's own exit-time teardown,
+# whose order against them is unspecified.
+_live_contexts: "weakref.WeakSet" = weakref.WeakSet()
+_live_dependents: "weakref.WeakSet" = weakref.WeakSet()
+_shut_down = False
+
+
+def _close_all_at_exit() -> None:
+    global _shut_down
+    for group in (_live_dependents, _live_contexts):
+        for obj in list(group):
+            try:
+                obj.close()
+            except Exception:
+                pass
+    _shut_down = True
+
+
 def _release(obj, _finalizing=sys.is_finalizing) -> None:
     """Destructor body: free the handle unless the interpreter is shutting
-    down (the HIP runtime may already be torn down; the process exit returns
-    the device memory anyway)."""
-    if _finalizing():
+    down (the exit hook has closed every live handle by then)."""
+    if _shut_down or _finalizing():
         return
     try:
         obj.close()
@@ -115,6 +137,7 @@ def lib() -> ctypes.CDLL:
         raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m minife_amd.build`")
     _share_torch_hip_runtime()
     L = ctypes.CDLL(LIB_PATH)
+    atexit.register(_close_all_at_exit)
     sig = {
         "mfe_ctx_create": ([ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
         "mfe_ctx_destroy": ([_vp], None),
@@ -209,6 +232,7 @@ class Context:
         if st != MFE_OK:
             raise MfeError(st, f"mfe_ctx_create(device={device}) failed (no GPU visible?)")
         self.h = h
+        _live_contexts.add(self)
         self.device = device
 
     def check(self, st: int, what: str = ""):
@@ -235,7 +259,12 @@ class Context:
         return ms.value
 
     def close(self):
+        """Destroy the context after what lives in it (their destroy calls
+        synchronize on this context's stream)."""
         if getattr(self, "h", None):
+            for obj in list(_live_dependents):
+                if getattr(obj, "ctx", None) is self:
+                    obj.close()
             lib().mfe_ctx_destroy(self.h)
             self.h = None
 
@@ -261,6 +290,7 @@ class CSRMatrix:
         h = _vp()
         self.ctx.check(lib().mfe_csr_create(self.ctx.h, ctypes.byref(h)), "csr_create")
         self.h = h
+        _live_dependents.add(self)
 
     @property
     def shape(self):
@@ -319,6 +349,7 @@ class Vector:
         h = _vp()
         self.ctx.check(lib().mfe_vec_create(self.ctx.h, int(n), ctypes.byref(h)), "vec_create")
         self.h = h
+        _live_dependents.add(self)
         self.n = int(n)
 
     @classmethod
@@ -333,6 +364,7 @@ class Vector:
         v.h = h
         v.n = int(n)
         v._owner = owner
+        _live_dependents.add(v)
         return v
 
     @classmethod
@@ -662,6 +694,7 @@ class Batch:
         self.ctx.check(lib().mfe_batch_create(self.ctx.h, int(self.nprob), _ptr(d), ctypes.byref(h)),
                        "batch_create")
         self.h = h
+        _live_dependents.add(self)
 
     def info(self, p: int = -1) -> dict:
         a = np.zeros(6, np.int64)
