@@ -954,11 +954,14 @@ __global__ __launch_bounds__(kBlock) void dot_seq_kernel(int64_t n, const double
 // Per iteration k (fast mode), all on one stream:
 //   cg_p       x += alpha_{k-1} p_{k-1} (deferred, see below); p = 1*r + beta*y
 //   spmv       Ap = A p, workgroup partials of p.Ap
-//   reduce     first stage of the p.Ap reduction
-//   alpha      p.Ap; breakdown test; alpha = rtrans / p.Ap
-//   norms(+fin) only when p.Ap < brkdown_tol: ||Ap||, ||p|| for breakdown()
-//   update     r = 1*r + (-alpha)*Ap, workgroup partials of r.r
-//   next       end of k; loop test; rtrans, beta, normr of k+1
+//   reduce     first stage of the p.Ap reduction (large n only)
+//   update     p.Ap; breakdown test; alpha = rtrans / p.Ap;
+//              r = 1*r + (-alpha)*Ap, workgroup partials of r.r (and, only
+//              when p.Ap < brkdown_tol, of ||Ap||^2 and ||p||^2)
+//   next       breakdown() if the norms were needed; end of k; loop test;
+//              rtrans, beta, normr of k+1
+// The per-operation timing mode runs the same sums as separate kernels
+// (reduce, alpha, norms, norms-fin, update, next).
 // The reference updates x right after r (cg_solve.hpp:211-212).  Here the
 // update x = 1*x + alpha*p of iteration k is applied by the next cg_p, which
 // reads p anyway (or by cg_finish after the last iteration): the same
@@ -1250,6 +1253,99 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(int64_t n, const CgSt
   }
 }
 
+// Up to this many groups of kStage SpMV partials, the fast iteration sums
+// p.Ap straight from the tile partials in each r-update workgroup (no
+// reduce_stage_kernel launch); beyond, from reduce_stage_kernel's group sums.
+constexpr int kInlineGroups = 4;
+
+// p.Ap in the order of reduce_stage_kernel + cg_alpha_fast_kernel: the tile
+// partials summed in groups of kStage by sum_partials, then the group sums by
+// sum_partials.  red2 != nullptr: the group sums are given (ng of them);
+// otherwise they are formed here from partials[0, np) (np <= kInlineGroups *
+// kStage).  Every workgroup that calls it gets the same bits.
+template <typename T>
+__device__ T pap_two_stage(const T* partials, int np, const T* red2, int ng, T* red) {
+  if (red2) return sum_partials<T>(red2, ng, red);
+  __shared__ T grp[kInlineGroups];
+  const int ngi = (int)cdiv(np, kStage);
+  for (int g = 0; g < ngi; ++g) {
+    const T sg = sum_partials<T>(partials + g * kStage, min(kStage, np - g * kStage), red);
+    if (threadIdx.x == 0) grp[g] = sg;
+  }
+  __syncthreads();
+  return sum_partials<T>(grp, ngi, red);
+}
+
+// The fast iteration's alpha step folded into the r update: every workgroup
+// forms p.Ap (pap_two_stage) and alpha = rtrans / p.Ap itself, workgroup 0
+// records them (cg_alpha_or_defer).  p.Ap < 0: breakdown, nothing updated.
+// p.Ap < brkdown_tol: the breakdown test's norms are needed; alpha does not
+// depend on them (cg_solve.hpp:186-201), so r is updated at once and the
+// partials of ||Ap||^2 and ||p||^2 go beside those of r.r for
+// cg_next_norms_kernel, which decides before anything else of the iteration
+// becomes visible (x is updated only once the iteration has ended, and r is
+// internal).  Partials: r.r at out[b], ||Ap||^2 at out[G + b], ||p||^2 at
+// out[2G + b] (G = the grid), the same per-workgroup sums as cg_update_kernel
+// and cg_norms_kernel.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_update_alpha_kernel(
+    int64_t n, CgState* st, T* __restrict__ r, const T* __restrict__ Ap, const T* __restrict__ p,
+    const T* partials, int np, const T* red2, int ng, double* hist, T* __restrict__ out) {
+  if (st->done) return;
+  __shared__ T red[4];
+  __shared__ double s_pap;
+  const double rtrans = st->rtrans, btol = st->brkdown_tol;
+  const T pap_t = pap_two_stage<T>(partials, np, red2, ng, red);  // valid in thread 0
+  if (threadIdx.x == 0) {
+    s_pap = (double)pap_t;
+    if (blockIdx.x == 0) cg_alpha_or_defer(st, (double)pap_t, hist);
+  }
+  __syncthreads();
+  const double pAp = s_pap;
+  if (pAp < btol && pAp < 0) return;
+  const bool need = pAp < btol;
+  const T malpha = (T)(-(rtrans / pAp));
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  T acc = T(0);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const T ri = T(1) * r[i] + malpha * Ap[i];
+    r[i] = ri;
+    acc += ri * ri;
+  }
+  acc = block_sum<T>(acc, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = acc;
+  if (!need) return;
+  const T a = block_sum<T>(grid_dot<T>(n, Ap, Ap), red);
+  const T b = block_sum<T>(grid_dot<T>(n, p, p), red);
+  if (threadIdx.x == 0) {
+    out[gridDim.x + blockIdx.x] = a;
+    out[2 * gridDim.x + blockIdx.x] = b;
+  }
+}
+
+// End of a fast iteration after cg_update_alpha_kernel: the breakdown test's
+// norms if they were needed (cg_alpha_after_norms: breakdown ends the solve
+// before the iteration counts), then cg_next with r.r.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cg_next_norms_kernel(CgState* st, const T* out, int G,
+                                                               double* hist) {
+  if (st->done) return;
+  __shared__ T red[4];
+  __shared__ int stop;
+  const T rr = sum_partials<T>(out, G, red);
+  if (st->need_norms) {
+    const T ApAp = sum_partials<T>(out + G, G, red);
+    const T pp = sum_partials<T>(out + 2 * G, G, red);
+    if (threadIdx.x == 0) {
+      cg_alpha_after_norms(st, (double)ApAp, (double)pp, hist);
+      stop = st->done;
+    }
+    __syncthreads();
+    if (stop) return;
+  }
+  if (threadIdx.x == 0) cg_next(st, (double)rr);
+}
+
 // --------------------------------------------------------- workspace -----
 
 // Attribution of GPU time to the reference's CG timer slots (WAXPY, DOT,
@@ -1302,7 +1398,8 @@ template <typename T>
 struct VecSet {
   int64_t n = -1;
   T *r = nullptr, *p = nullptr, *Ap = nullptr;
-  T* partials = nullptr;  // 3 x max(tiles, kVecBlocks)
+  T* partials = nullptr;  // 4 x max(tiles, kVecBlocks)
+  int64_t part_base = 0;  // max(tiles, kVecBlocks): the fast iteration's second region
   T* red2 = nullptr;      // ceil(tiles / kStage)
   void release() {
     hipFree(r);
@@ -1368,8 +1465,8 @@ static int vecset_prepare(mfe_ctx* ctx, VecSet<T>& v, int64_t n, GraphSlot& g) {
   MFE_CHECK(ctx, hipMalloc(&v.r, bytes));
   MFE_CHECK(ctx, hipMalloc(&v.p, bytes));
   MFE_CHECK(ctx, hipMalloc(&v.Ap, bytes));
-  const int64_t np = 3 * (tiles > kVecBlocks ? tiles : kVecBlocks);
-  MFE_CHECK(ctx, hipMalloc(&v.partials, sizeof(T) * (size_t)np));
+  v.part_base = tiles > kVecBlocks ? tiles : kVecBlocks;
+  MFE_CHECK(ctx, hipMalloc(&v.partials, sizeof(T) * 4 * (size_t)v.part_base));
   const int64_t n2 = cdiv(tiles, kStage) > 0 ? cdiv(tiles, kStage) : 1;
   MFE_CHECK(ctx, hipMalloc(&v.red2, sizeof(T) * (size_t)n2));
   v.n = n;
@@ -1434,15 +1531,29 @@ static hipError_t enqueue_iteration(const SpmvOp<T>& op, VecSet<T>& v, CgState* 
   launch_spmv<T, false, kDot>(op, v.p, v.Ap, nullptr, v.partials, done, s, &tiles);
   mark(tm, MFE_T_MATVEC);
   const int nb2 = (int)cdiv(tiles, kStage);
-  reduce_stage_kernel<T><<<nb2, kBlock, 0, s>>>(v.partials, tiles, 1, v.red2, done);
-  cg_alpha_fast_kernel<T><<<1, kBlock, 0, s>>>(st, v.red2, nb2, hist);
-  cg_norms_kernel<T><<<vg, kBlock, 0, s>>>(n, st, v.Ap, v.p, v.partials);
-  cg_norms_fin_kernel<T><<<1, kBlock, 0, s>>>(st, v.partials, (int)vg, hist);
-  mark(tm, MFE_T_DOT);
-  cg_update_kernel<T, true><<<vg, kBlock, 0, s>>>(n, st, v.r, v.Ap, v.partials);
-  mark(tm, MFE_T_WAXPY);
-  cg_next_fast_kernel<T><<<1, kBlock, 0, s>>>(st, v.partials, (int)vg);
-  mark(tm, MFE_T_DOT);
+  if (tm) {
+    // Timed per operation: the dot products stay kernels of their own.
+    reduce_stage_kernel<T><<<nb2, kBlock, 0, s>>>(v.partials, tiles, 1, v.red2, done);
+    cg_alpha_fast_kernel<T><<<1, kBlock, 0, s>>>(st, v.red2, nb2, hist);
+    cg_norms_kernel<T><<<vg, kBlock, 0, s>>>(n, st, v.Ap, v.p, v.partials);
+    cg_norms_fin_kernel<T><<<1, kBlock, 0, s>>>(st, v.partials, (int)vg, hist);
+    mark(tm, MFE_T_DOT);
+    cg_update_kernel<T, true><<<vg, kBlock, 0, s>>>(n, st, v.r, v.Ap, v.partials);
+    mark(tm, MFE_T_WAXPY);
+    cg_next_fast_kernel<T><<<1, kBlock, 0, s>>>(st, v.partials, (int)vg);
+    mark(tm, MFE_T_DOT);
+    return hipGetLastError();
+  }
+  // Same sums in the same order as above, in 3 to 4 launches instead of 6.
+  // The r update's partials sit past the SpMV's (which it may read).
+  T* out = v.partials + v.part_base;
+  const bool inline_groups = nb2 <= kInlineGroups;
+  if (!inline_groups)
+    reduce_stage_kernel<T><<<nb2, kBlock, 0, s>>>(v.partials, tiles, 1, v.red2, done);
+  cg_update_alpha_kernel<T><<<vg, kBlock, 0, s>>>(n, st, v.r, v.Ap, v.p, v.partials, tiles,
+                                                  inline_groups ? nullptr : v.red2, nb2, hist,
+                                                  out);
+  cg_next_norms_kernel<T><<<1, kBlock, 0, s>>>(st, out, (int)vg, hist);
   return hipGetLastError();
 }
 

@@ -617,6 +617,44 @@ def check_fast_history(ours, ref, n, kmax=7):
         " ".join(f"{w:.1e}" for w in worst))
 
 
+@pytest.mark.parametrize("replay", [False, True])
+def test_cg_norms_branch_is_scale_exact(mfe, ctx, oracle_mod, replay):
+    """b scaled by s = 2^-40 makes p.Ap < DBL_EPSILON from iteration 1, so the
+    breakdown test's norms branch (cg_solve.hpp:186-200) runs at iteration 1
+    and whenever p.Ap falls below a tenth of its last tested value, and the
+    solve continues with brkdown_tol = 0.1 p.Ap.  (Unscaled, p.Ap stays above
+    DBL_EPSILON through iteration 18 at 10^3, so that solve never takes it.)  Power-of-
+    two scaling is exact in binary floating point: x, r, p scale by s, rtrans
+    and p.Ap by s^2, alpha and beta not at all.  So the scaled solve must be the
+    unscaled one, bit for bit, scaled -- which pins the branch in fast mode
+    (alpha after the norms, the new tolerance) as tightly as replay mode's
+    oracle comparison does.  tolerance 0 keeps the absolute stopping test from
+    ending the scaled solve early."""
+    s = 2.0 ** -40
+    mesh, A, b = mfe.build_problem(10, ctx=ctx)
+    bh = b.to_numpy()
+    bs = mfe.Vector.from_numpy(bh * s, ctx)
+    x1 = mfe.Vector(A.nrows, ctx)
+    xs = mfe.Vector(A.nrows, ctx)
+    kw = dict(max_iter=18, tolerance=0.0, replay=replay, history=True)
+    r1 = mfe.cg_solve(A, b, x1, **kw)
+    rs = mfe.cg_solve(A, bs, xs, **kw)
+    assert r1.status == mfe.MFE_OK and rs.status == mfe.MFE_OK
+    assert r1.num_iters == rs.num_iters == 18
+    h1, hs = r1.history[:18], rs.history[:18]
+    assert np.all(hs[:, 1] < EPS) and np.all(h1[:, 1] > EPS)
+    assert bits_equal(hs[:, 0], h1[:, 0] * s * s) and bits_equal(hs[:, 1], h1[:, 1] * s * s)
+    assert bits_equal(hs[:, 2], h1[:, 2]) and bits_equal(hs[:, 3], h1[:, 3])
+    assert bits_equal(xs.to_numpy(), x1.to_numpy() * s)
+    assert rs.normr == r1.normr * s
+    if replay:
+        P = oracle_mod.build_problem(10, 10, 10)
+        ref = oracle_mod.cg_solve(P["A"], P["b"] * s, max_iter=18, tol=0.0)
+        assert ref["num_iters"] == 18
+        assert bits_equal(xs.to_numpy(), ref["x"])
+        assert bits_equal(hs, ref["history"])
+
+
 def test_cg_fast_mode_30(mfe, ctx):
     """Fast mode: tree-reduced dots.  Scalar history close to the reference's
     over iterations 1-7 (check_fast_history) and the converged solution within
@@ -690,6 +728,25 @@ def test_cg_breakdown_on_indefinite_operator(mfe, ctx):
     x = mfe.Vector(n, ctx)
     r = mfe.cg_solve(A, b, x, max_iter=10)
     assert r.breakdown and r.num_iters == 0
+
+
+@pytest.mark.parametrize("replay", [False, True])
+def test_cg_breakdown_by_norms_test(mfe, ctx, oracle_mod, replay):
+    """The swap operator [[0,1],[1,0]] with b = e_1 gives p.Ap = 0 exactly:
+    not negative, but |p.Ap| <= 100 |Ap| |p| eps (cg_solve.hpp:49-67), so the
+    norms test reports breakdown at iteration 1, as the oracle does."""
+    A = mfe.CSRMatrix.from_host([0, 1, 2], [1, 0], [1.0, 1.0], ctx=ctx)
+    mfe.make_local_matrix(A)
+    b = mfe.Vector.from_numpy([1.0, 0.0], ctx)
+    x = mfe.Vector(2, ctx)
+    r = mfe.cg_solve(A, b, x, max_iter=5, replay=replay, history=True)
+    swap = oracle_mod.CSR(np.arange(2, dtype=np.int32), np.array([0, 1, 2], np.int32),
+                          np.array([1, 0], np.int32), np.array([1.0, 1.0]))
+    ref = oracle_mod.cg_solve(swap, np.array([1.0, 0.0]), max_iter=5)
+    assert ref["status"] == 1 and ref["num_iters"] == 0
+    assert r.breakdown and r.num_iters == 0
+    assert r.history[0, 1] == 0.0 and r.history[0, 2] == 0.0
+    assert list(x.to_numpy()) == [0.0, 0.0]
 
 
 # ----------------------------------------------------------- full size ------

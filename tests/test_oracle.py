@@ -335,6 +335,28 @@ def test_cg_invariants(oracle_mod):
     assert np.all(hist[:, 2] > 0) and np.all(hist[:, 1] > 0)
 
 
+def test_cg_norms_branch_scale_exact(oracle_mod):
+    """b scaled by 2^-40 puts p.Ap below DBL_EPSILON from iteration 1, so the
+    breakdown test's norms branch (cg_solve.hpp:186-200) runs; power-of-two
+    scaling is exact, so the solve must be the unscaled one scaled bit for bit
+    (alpha, beta unchanged; rtrans, p.Ap by 2^-80; x by 2^-40).  A branch
+    that skipped alpha or mis-set the new tolerance would break it.  The
+    2x2 swap operator gives p.Ap = 0 exactly: breakdown by the norms test."""
+    s = 2.0 ** -40
+    P = oracle_mod.build_problem(10, 10, 10)
+    r = oracle_mod.cg_solve(P["A"], P["b"], max_iter=18, tol=0.0)
+    rs = oracle_mod.cg_solve(P["A"], P["b"] * s, max_iter=18, tol=0.0)
+    h, hs = r["history"], rs["history"]
+    assert r["num_iters"] == rs["num_iters"] == 18 and rs["status"] == 0
+    assert np.all(hs[:, 1] < EPS) and np.all(h[:, 1] > EPS)
+    assert bits_equal(hs[:, :2], h[:, :2] * s * s) and bits_equal(hs[:, 2:], h[:, 2:])
+    assert bits_equal(rs["x"], r["x"] * s) and rs["normr"] == r["normr"] * s
+    swap = oracle_mod.CSR(np.arange(2, dtype=np.int32), np.array([0, 1, 2], np.int32),
+                          np.array([1, 0], np.int32), np.array([1.0, 1.0]))
+    rb = oracle_mod.cg_solve(swap, np.array([1.0, 0.0]), max_iter=5)
+    assert rb["status"] == 1 and rb["num_iters"] == 0
+
+
 def test_analytic_solution(oracle_mod):
     # Laplace problem with u=1 on one face of the cube: centre value 1/6 by
     # symmetry (superposing the six rotated problems gives u = 1).
