@@ -345,12 +345,29 @@ def main():
     tol_run = None
     if not args.no_tol_run:
         bnorm = float(np.sqrt(mfe.dot(b, b)))
-        # FP64 plain CG to ||r||/||b|| <= 1e-10 (warm run first, then timed)
+        res = mfe.Vector(n, ctx)
+
+        def true_rel(xv):
+            # ||b - A x|| / ||b|| in FP64 on the GPU
+            mfe.matvec(A, xv, res)
+            mfe.waxpby(1.0, b, -1.0, res, res)
+            return float(np.sqrt(mfe.dot(res, res))) / bnorm
+
+        # FP64 plain CG to ||r||/||b|| <= 1e-10 (warm run first, then timed).  CG
+        # stops on its recursive residual; when the true one is still above the
+        # target the solve is restarted from the current x (cg_solve's first step
+        # recomputes r = b - A x), as a user needing the true residual would.
         for timed in (False, True):
             x.fill(0.0)
             ctx.synchronize()
             t0 = time.perf_counter()
-            r2 = mfe.cg_solve(A, b, x, max_iter=5000, tolerance=1e-10, relative=True)
+            its64, restarts = 0, 0
+            while True:
+                r2 = mfe.cg_solve(A, b, x, max_iter=5000, tolerance=1e-10, relative=True)
+                its64 += r2.num_iters
+                if r2.num_iters == 0 or true_rel(x) <= 1e-10 or restarts >= 5:
+                    break
+                restarts += 1
             tt = time.perf_counter() - t0
         x64 = x.to_numpy()
         # mixed: FP32 inner CG to 1e-5 + FP64 refinement to 1e-10
@@ -373,7 +390,6 @@ def main():
             tk = time.perf_counter() - t0
         xkh = xk.to_numpy()
         # all residuals recomputed in FP64 on the GPU
-        res = mfe.Vector(n, ctx)
         bh = b.to_numpy()
         mfe.matvec(A, x, res)
         rel64 = float(np.linalg.norm(bh - res.to_numpy())) / bnorm
@@ -383,7 +399,8 @@ def main():
         relk = float(np.linalg.norm(bh - res.to_numpy())) / bnorm
         err = float(np.linalg.norm(xmh - x64) / np.linalg.norm(x64))
         errk = float(np.linalg.norm(xkh - x64) / np.linalg.norm(x64))
-        tol_run = {"fp64_s": tt, "fp64_iters": r2.num_iters, "fp64_true_rel_residual": rel64,
+        tol_run = {"fp64_s": tt, "fp64_iters": its64, "fp64_restarts": restarts,
+                   "fp64_true_rel_residual": rel64,
                    "mixed_s": tm, "mixed_outer": rm.outer_iters, "mixed_inner": rm.inner_iters,
                    "mixed_true_rel_residual": relm, "mixed_converged": rm.converged,
                    "mixed_rel_err_vs_fp64": err, "speedup_mixed_vs_fp64": tt / tm,
@@ -391,7 +408,8 @@ def main():
                        "inner_tol": 0.1, "s": tk, "outer": rk.outer_iters, "inner": rk.inner_iters,
                        "true_rel_residual": relk, "converged": rk.converged,
                        "rel_err_vs_fp64": errk, "speedup_vs_fp64": tt / tk}}
-        log(f"time to 1e-10: fp64 {tt:.3f}s ({r2.num_iters} its, true rel {rel64:.2e}); mixed "
+        log(f"time to 1e-10: fp64 {tt:.3f}s ({its64} its, {restarts} restarts, true rel "
+            f"{rel64:.2e}); mixed "
             f"{tm:.3f}s ({rm.outer_iters} outer, {rm.inner_iters} inner, true rel {relm:.2e}), "
             f"|x_mixed-x_fp64|/|x_fp64| = {err:.2e}; keep-direction {tk:.3f}s ({rk.outer_iters} "
             f"outer, {rk.inner_iters} inner, true rel {relk:.2e}, err {errk:.2e})")
