@@ -1500,9 +1500,12 @@ static unsigned vec_grid(int64_t n) {
   return (unsigned)(b < kVecBlocks ? (b > 0 ? b : 1) : kVecBlocks);
 }
 
-// Element-wise CG kernels without a sum run one element per thread: measured
-// at n = 27.3 M, the p update moves its 40n bytes in 181 us this way against
-// 199-222 us for grid-stride loops over 1,024-16,384 workgroups.
+// Grids of the FP64 CG's vector kernels.  The p update (no sum) runs one
+// element per thread: measured at n = 27.3 M it moves its 40n bytes in 181 us
+// this way against 199-222 us for grid-stride loops over 1,024-16,384
+// workgroups.  (The FP32 CG of the mixed solve measured faster with the
+// 2,048-workgroup grid-stride form for both kernels: 0.917 against 0.947 ms
+// per inner iteration at 300^3, so it keeps vec_grid.)
 static unsigned stream_grid(int64_t n) {
   const int64_t b = cdiv(n, kBlock);
   return (unsigned)(b > 0 ? b : 1);
@@ -1527,8 +1530,9 @@ static hipError_t enqueue_iteration(const SpmvOp<T>& op, VecSet<T>& v, CgState* 
                                     double* hist, T* x, const T* y_for_p, bool replay,
                                     hipStream_t s, OpTimer* tm = nullptr) {
   const int64_t n = v.n;
-  const unsigned vg = upd_grid(n);
-  cg_p_kernel<T><<<stream_grid(n), kBlock, 0, s>>>(n, st, v.r, y_for_p, v.p, x);
+  constexpr bool kF64 = sizeof(T) == 8;
+  const unsigned vg = kF64 ? upd_grid(n) : vec_grid(n);
+  cg_p_kernel<T><<<kF64 ? stream_grid(n) : vg, kBlock, 0, s>>>(n, st, v.r, y_for_p, v.p, x);
   mark(tm, MFE_T_WAXPY);
   int tiles = 0;
   const int* done = &st->done;
