@@ -804,5 +804,14 @@ def test_full_size_300(mfe, ctx):
     true_rel = np.linalg.norm(bh - y.to_numpy()) / np.linalg.norm(bh)
     say(f"fast CG: {r3.num_iters} iterations, recursive {r3.normr / np.linalg.norm(bh):.3e}, "
         f"true relative residual {true_rel:.3e}")
-    assert r3.normr <= 1e-10 * np.linalg.norm(bh) and true_rel <= 2e-10
+    # CG stops on its recursive residual; the true one differs by the rounding
+    # the r and x updates accumulate (the residual gap), which moves with the
+    # summation order of the dots: 1e-10 to 7e-10 here.  A restart from x
+    # (r recomputed as b - A x) brings the true residual under the target.
+    assert r3.normr <= 1e-10 * np.linalg.norm(bh) and true_rel <= 1e-9
+    r4 = mfe.cg_solve(A, b, x2, max_iter=5000, tolerance=1e-10, relative=True)
+    mfe.matvec(A, x2, y)
+    true_rel2 = np.linalg.norm(bh - y.to_numpy()) / np.linalg.norm(bh)
+    say(f"after a restart: {r4.num_iters} more iterations, true relative residual {true_rel2:.3e}")
+    assert true_rel2 <= 1e-10
     assert mfe.verify_solution(mesh, x2)[0] == 0
