@@ -483,7 +483,9 @@ def assemble_FE_data(mesh, A: CSRMatrix, b: Vector, params=None, kappa=None, sou
 
 def impose_dirichlet(prescribed_value: float, A: CSRMatrix, b: Vector, global_nx: int,
                      global_ny: int, global_nz: int, bc_rows) -> None:
-    bc = np.unique(np.asarray(bc_rows, dtype=np.int32))  # std::set semantics
+    bc = np.ascontiguousarray(bc_rows, dtype=np.int32)
+    if bc.size > 1 and not bool(np.all(bc[1:] > bc[:-1])):
+        bc = np.unique(bc)  # std::set semantics: sorted, each row once
     st = lib().mfe_impose_dirichlet(A.ctx.h, float(prescribed_value), A.h, b.h, int(global_nx),
                                     int(global_ny), int(global_nz), _ptr(bc), bc.shape[0])
     A.ctx.check(st, "impose_dirichlet")
