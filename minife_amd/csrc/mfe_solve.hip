@@ -252,10 +252,116 @@ __device__ __forceinline__ int2 block_scan2(int a, int b, int* sa, int* sb, int&
   return make_int2(sa[t] - a, sb[t] - b);
 }
 
-// One workgroup per tile: the tile's columns sorted in LDS (bitonic); a value
-// is distinct where it differs from its predecessor and starts a run where the
-// predecessor is not value - 1.  Pass 0: counts[tile] = {distinct, runs}, or
-// {-1, -1} past kXCap non-zeros.  Pass 1: the runs at roff[tile], the local
+// Column range (in columns) up to which a tile's distinct columns are found
+// with a bitmap in LDS instead of a sort: 2^18 bits, the sort's 32 KB.
+constexpr int kBmWords = kXCap;
+constexpr int64_t kBmRange = 32 * (int64_t)kBmWords;
+
+// The slot loop of a plan's pass 1 (shared by both ways of listing the
+// distinct columns): each lane's local column per slot, and per slice whether
+// the slot's local columns are base_j + lane (saff, sbase).
+template <typename Loc>
+__device__ __forceinline__ void sellx_plan_slots(int n, int nslices, const int* __restrict__ ro,
+                                                 const int* __restrict__ cols,
+                                                 const int* __restrict__ soff,
+                                                 const unsigned char* __restrict__ slen,
+                                                 unsigned short* __restrict__ lcols,
+                                                 unsigned char* __restrict__ saff,
+                                                 unsigned short* __restrict__ sbase, Loc loc_of) {
+  const int t = (int)threadIdx.x, tile = (int)blockIdx.x;
+  const int slice = tile * kSlicesPerBlock + t / kSlice;
+  if (slice >= nslices) return;
+  const int row = tile * kSpmvRows + t;
+  const int lane = t % kSlice;
+  const int o = soff[slice], w = soff[slice + 1] - o;
+  const int len = slen[row];
+  const int start = row < n ? ro[row] : 0;
+  bool affine = w > 0 && len == w;
+  for (int j = 0; j < w; ++j) {  // w is uniform over the wave
+    const int loc = j < len ? loc_of(cols[start + j]) : 0;
+    lcols[sell_pos((int64_t)o * kSlice, j, w, lane)] = (unsigned short)loc;
+    const int base = __shfl(loc, 0, kWave);
+    affine = affine && loc == base + lane;
+    if (lane == 0) sbase[(int64_t)slice * kSellBaseStride + j] = (unsigned short)base;
+  }
+  if (lane >= w && lane < kSellBaseStride) sbase[(int64_t)slice * kSellBaseStride + lane] = 0;
+  const int all = __all(affine ? 1 : 0);
+  if (lane == 0) saff[slice] = (unsigned char)(all ? 1 : 0);
+}
+
+// Bitmap path of sellx_plan_kernel (see there): bm = kBmWords words of LDS,
+// bit c - cmin set for every column c of the tile.  Thread t owns words
+// [t * kPer, (t + 1) * kPer).
+__device__ void sellx_plan_bitmap(int n, int nslices, const int* __restrict__ ro,
+                                  const int* __restrict__ cols, const int* __restrict__ soff,
+                                  const unsigned char* __restrict__ slen, int pass,
+                                  int2* __restrict__ counts, const int* __restrict__ roff,
+                                  int* __restrict__ rcol, int* __restrict__ rpos,
+                                  unsigned short* __restrict__ lcols,
+                                  unsigned char* __restrict__ saff,
+                                  unsigned short* __restrict__ sbase, unsigned* bm, int* gpre,
+                                  int* sa, int* sb, int s0, int cnt, int cmin) {
+  constexpr int kPer = kBmWords / kBlock;  // 32 words per thread = 4 groups of 8
+  const int t = (int)threadIdx.x, tile = (int)blockIdx.x;
+  for (int i = t; i < kBmWords; i += kBlock) bm[i] = 0u;
+  __syncthreads();
+  for (int i = t; i < cnt; i += kBlock) {
+    const int d = cols[s0 + i] - cmin;
+    atomicOr(&bm[d >> 5], 1u << (d & 31));
+  }
+  __syncthreads();
+  const int w0 = t * kPer;
+  unsigned prev_msb = w0 > 0 ? bm[w0 - 1] >> 31 : 0u;
+  int nu = 0, nr = 0;
+  for (int q = 0; q < kPer; ++q) {
+    const unsigned wd = bm[w0 + q];
+    const unsigned st = wd & ~((wd << 1) | prev_msb);  // set bits whose predecessor is clear
+    nu += __popc(wd);
+    nr += __popc(st);
+    prev_msb = wd >> 31;
+  }
+  int tu, tr;
+  const int2 base = block_scan2(nu, nr, sa, sb, tu, tr);
+  if (pass == 0) {
+    if (t == 0) counts[tile] = make_int2(tu, tr);
+    return;
+  }
+  // runs in increasing column order, and the set bits before each group of 8 words
+  int u = base.x, r = base.y;
+  const int rb = roff[tile];
+  prev_msb = w0 > 0 ? bm[w0 - 1] >> 31 : 0u;
+  for (int q = 0; q < kPer; ++q) {
+    if ((q & 7) == 0) gpre[(w0 + q) >> 3] = u;
+    const unsigned wd = bm[w0 + q];
+    unsigned st = wd & ~((wd << 1) | prev_msb);
+    while (st) {
+      const int bit = __ffs(st) - 1;
+      rcol[rb + r] = cmin + (w0 + q) * 32 + bit;
+      rpos[rb + r] = u + __popc(wd & ((1u << bit) - 1u));
+      ++r;
+      st &= st - 1u;
+    }
+    u += __popc(wd);
+    prev_msb = wd >> 31;
+  }
+  __syncthreads();
+  sellx_plan_slots(n, nslices, ro, cols, soff, slen, lcols, saff, sbase, [&](int c) {
+    const int d = c - cmin, wi = d >> 5;
+    int p = gpre[wi >> 3];
+    for (int k = wi & ~7; k < wi; ++k) p += __popc(bm[k]);
+    return p + __popc(bm[wi] & ((1u << (d & 31)) - 1u));
+  });
+}
+
+// One workgroup per tile.  The tile's distinct columns, in increasing order,
+// come from a bitmap over [min column, max column] when that range is at most
+// kBmRange (a mesh numbering's tiles span about two planes): each entry sets
+// its column's bit, a set bit is a distinct column, a set bit after a clear one
+// starts a run, and a column's position in the list is the number of set bits
+// before it.  Wider tiles sort their columns in LDS (bitonic); a value is
+// distinct where it differs from its predecessor and starts a run where the
+// predecessor is not value - 1.  Both give the same lists.  Pass 0:
+// counts[tile] = {distinct, runs}, or {-1, -1} past kXCap non-zeros.  Pass 1: the runs at roff[tile], the local
 // columns at every slot of the tile's slices (padding slots: 0), and per slice
 // (one wave) whether every row has the slice's width w > 0 and slot j's local
 // column is base_j + lane for every j (saff = 1, bases in sbase; 0 past w).
@@ -267,12 +373,39 @@ __global__ __launch_bounds__(kBlock) void sellx_plan_kernel(
     unsigned short* __restrict__ sbase) {
   __shared__ int a[kXCap];
   __shared__ int sa[kBlock], sb[kBlock];
+  __shared__ int gpre[kBmWords / 8];  // bitmap path: set bits before each group of 8 words
   constexpr int kPer = kXCap / kBlock;
   const int t = (int)threadIdx.x, tile = (int)blockIdx.x;
   const int r0 = tile * kSpmvRows, r1 = min(r0 + kSpmvRows, n);
   const int s0 = ro[r0], cnt = ro[r1] - s0;
   if (cnt > kXCap) {
     if (pass == 0 && t == 0) counts[tile] = make_int2(-1, -1);
+    return;
+  }
+  // the tile's column range
+  int cmin = INT_MAX, cmax = -1;
+  for (int i = t; i < cnt; i += kBlock) {
+    const int c = cols[s0 + i];
+    cmin = min(cmin, c);
+    cmax = max(cmax, c);
+  }
+  sa[t] = cmin;
+  sb[t] = cmax;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (t < off) {
+      sa[t] = min(sa[t], sa[t + off]);
+      sb[t] = max(sb[t], sb[t + off]);
+    }
+    __syncthreads();
+  }
+  cmin = sa[0];
+  cmax = sb[0];
+  __syncthreads();
+  if (cnt == 0 || (int64_t)cmax - cmin + 1 <= kBmRange) {
+    sellx_plan_bitmap(n, nslices, ro, cols, soff, slen, pass, counts, roff, rcol, rpos, lcols,
+                      saff, sbase, reinterpret_cast<unsigned*>(a), gpre, sa, sb, s0, cnt,
+                      cnt == 0 ? 0 : cmin);
     return;
   }
   for (int i = t; i < kXCap; i += kBlock) a[i] = i < cnt ? cols[s0 + i] : INT_MAX;
@@ -327,34 +460,15 @@ __global__ __launch_bounds__(kBlock) void sellx_plan_kernel(
       ++u;
     }
   __syncthreads();
-  const int slice = tile * kSlicesPerBlock + t / kSlice;
-  if (slice >= nslices) return;
-  const int row = tile * kSpmvRows + t;
-  const int lane = t % kSlice;
-  const int o = soff[slice], w = soff[slice + 1] - o;
-  const int len = slen[row];
-  const int start = row < n ? ro[row] : 0;
-  bool affine = w > 0 && len == w;
-  for (int j = 0; j < w; ++j) {  // w is uniform over the wave
-    int loc = 0;
-    if (j < len) {
-      const int c = cols[start + j];
-      int lo = 0, hi = tu;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (a[mid] < c) lo = mid + 1;
-        else hi = mid;
-      }
-      loc = lo;
+  sellx_plan_slots(n, nslices, ro, cols, soff, slen, lcols, saff, sbase, [&](int c) {
+    int lo = 0, hi = tu;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (a[mid] < c) lo = mid + 1;
+      else hi = mid;
     }
-    lcols[sell_pos((int64_t)o * kSlice, j, w, lane)] = (unsigned short)loc;
-    const int base = __shfl(loc, 0, kWave);
-    affine = affine && loc == base + lane;
-    if (lane == 0) sbase[(int64_t)slice * kSellBaseStride + j] = (unsigned short)base;
-  }
-  if (lane >= w && lane < kSellBaseStride) sbase[(int64_t)slice * kSellBaseStride + lane] = 0;
-  const int all = __all(affine ? 1 : 0);
-  if (lane == 0) saff[slice] = (unsigned char)(all ? 1 : 0);
+    return lo;
+  });
 }
 
 // spmv_sell_kernel's modes and results, x staged per tile (dynamic LDS:
