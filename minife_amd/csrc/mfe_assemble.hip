@@ -157,29 +157,55 @@ __global__ __launch_bounds__(kBlock) void structure_kernel(int gx, int gy, int g
                                                            int* __restrict__ rows,
                                                            int* __restrict__ ro,
                                                            int* __restrict__ cols) {
+  // A row's (up to 27) columns are written to LDS first and the workgroup's
+  // 256 rows of columns, contiguous in cols, are stored with coalesced writes
+  // (rows are 108 bytes apart, so per-thread stores would touch 64 cache
+  // lines per instruction).
+  __shared__ int sc[kSpmvMaxNnz];
+  __shared__ int64_t s_end;
   const int64_t n = (int64_t)gx * gy * gz;
-  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= n) return;
-  const int ix = (int)(id % gx);
-  const int iy = (int)((id / gx) % gy);
-  const int iz = (int)(id / ((int64_t)gx * gy));
+  const int64_t r0 = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t id = r0 + threadIdx.x;
   const int64_t Sx = 3 * (int64_t)gx - 2, Sy = 3 * (int64_t)gy - 2;
-  const int64_t cz = axis_cnt(iz, gz), cy = axis_cnt(iy, gy);
-  const int64_t off = axis_pre(iz) * Sy * Sx + cz * axis_pre(iy) * Sx + cz * cy * axis_pre(ix);
-  rows[id] = (int)id;
-  ro[id] = (int)off;
-  if (id == n - 1) ro[n] = (int)(off + cz * cy * axis_cnt(ix, gx));
-  int64_t pos = off;
+  const int64_t last = (r0 + kBlock < n ? r0 + kBlock : n) - 1;
+  int64_t off = 0, s0 = 0;
+  if (id <= last) {
+    const int ix = (int)(id % gx);
+    const int iy = (int)((id / gx) % gy);
+    const int iz = (int)(id / ((int64_t)gx * gy));
+    const int64_t cz = axis_cnt(iz, gz), cy = axis_cnt(iy, gy);
+    off = axis_pre(iz) * Sy * Sx + cz * axis_pre(iy) * Sx + cz * cy * axis_pre(ix);
+    const int64_t len = cz * cy * axis_cnt(ix, gx);
+    rows[id] = (int)id;
+    ro[id] = (int)off;
+    if (id == n - 1) ro[n] = (int)(off + len);
+    if (id == last) s_end = off + len;
+  }
+  // the tile's first offset (thread 0's row), shared through LDS; a tile holds
+  // at most 27 * 256 = kSpmvMaxNnz columns
+  __shared__ int64_t s_begin;
+  if (threadIdx.x == 0) s_begin = off;
+  __syncthreads();
+  s0 = s_begin;
+  if (id <= last) {
+    const int ix = (int)(id % gx);
+    const int iy = (int)((id / gx) % gy);
+    const int iz = (int)(id / ((int64_t)gx * gy));
+    int pos = (int)(off - s0);
 #pragma unroll
-  for (int sz = -1; sz <= 1; ++sz)
+    for (int sz = -1; sz <= 1; ++sz)
 #pragma unroll
-    for (int sy = -1; sy <= 1; ++sy)
+      for (int sy = -1; sy <= 1; ++sy)
 #pragma unroll
-      for (int sx = -1; sx <= 1; ++sx) {
-        const int x = ix + sx, y = iy + sy, z = iz + sz;
-        if (x < 0 || y < 0 || z < 0 || x >= gx || y >= gy || z >= gz) continue;
-        cols[pos++] = (int)(x + (int64_t)gx * y + (int64_t)gx * gy * z);
-      }
+        for (int sx = -1; sx <= 1; ++sx) {
+          const int x = ix + sx, y = iy + sy, z = iz + sz;
+          if (x < 0 || y < 0 || z < 0 || x >= gx || y >= gy || z >= gz) continue;
+          sc[pos++] = (int)(x + (int64_t)gx * y + (int64_t)gx * gy * z);
+        }
+  }
+  __syncthreads();
+  const int cnt = (int)(s_end - s0);
+  for (int i = (int)threadIdx.x; i < cnt; i += kBlock) cols[s0 + i] = sc[i];
 }
 
 __global__ void bc_flag_kernel(int64_t nbc, const int* __restrict__ bc, int64_t n,
