@@ -461,6 +461,72 @@ def test_staged_x_on_banded_and_scattered_columns(mfe, ctx, oracle_mod, band, ke
     assert bits_equal(y.to_numpy(), oracle_mod.matvec_f32(Ao, xh))
 
 
+@pytest.mark.parametrize("maxlen", [27, 80])
+def test_dirichlet_on_uploaded_matrices(mfe, ctx, oracle_mod, maxlen):
+    """impose_dirichlet on general CSR matrices (random columns, some empty
+    rows, zero values): with at most 27 entries per row every 256-row tile
+    fits the tiled kernel (columns staged in LDS); rows of up to 80 entries
+    overflow it and take the per-row kernel.  Both equal the oracle's values
+    and right-hand side bit for bit, for value 0 and 1."""
+    say(f"case dirichlet uploaded maxlen={maxlen}")
+    rng = np.random.default_rng(maxlen)
+    n = 3000
+    ro, cols, vals = _ragged_csr(rng, n, maxlen)
+    bh = rng.standard_normal(n)
+    bc = np.sort(rng.choice(n, size=400, replace=False)).astype(np.int32)
+    Ao = oracle_mod.CSR(np.arange(n, dtype=np.int32), ro, cols, vals.copy())
+    bo = bh.copy()
+    A = mfe.CSRMatrix.from_host(ro, cols, vals, ctx=ctx)
+    b = mfe.Vector.from_numpy(bh, ctx)
+    for value, rows in ((0.0, bc[:300]), (1.0, bc[300:])):
+        mfe.impose_dirichlet(value, A, b, 0, 0, 0, rows)
+        oracle_mod.impose_dirichlet(value, Ao, bo, rows)
+        assert bits_equal(A.to_host(rows=False, row_offsets=False, cols=False)["vals"], Ao.vals)
+        assert bits_equal(b.to_numpy(), bo)
+
+
+def test_staged_x_plan_bitmap_and_sort_tiles(mfe, ctx, oracle_mod):
+    """The plan lists a tile's distinct columns with an LDS bitmap when they
+    span at most 2^18 columns and by sorting otherwise.  400,000 rows of a
+    3-point band, where the even tiles of the first and last 100,000 rows also
+    reach a column 300,000 away (a span above 2^18: sorted) and every other
+    tile stays within its band (bitmap): the plan's counts match the numpy
+    model, and SpMV, SpMV+dot and the FP32 SpMV equal the oracle bit for bit."""
+    say("case staged x bitmap and sort tiles")
+    n, far = 400_000, 300_000
+    rng = np.random.default_rng(23)
+    parts = []
+    for r in range(n):
+        c = [v for v in (r - 1, r, r + 1) if 0 <= v < n]
+        if (r // 256) % 2 == 0:
+            if r + far < n:
+                c.append(r + far)
+            elif r - far >= 0:
+                c.insert(0, r - far)
+        parts.append(c)
+    ro = np.concatenate([[0], np.cumsum([len(c) for c in parts])]).astype(np.int32)
+    cols = np.fromiter((v for c in parts for v in c), dtype=np.int32, count=int(ro[-1]))
+    vals = rng.standard_normal(cols.shape[0])
+    assert expected_auto_kernel(ro, cols, n) == "sell64-staged-x"
+    Ao = oracle_mod.CSR(np.arange(n, dtype=np.int32), ro, cols, vals)
+    A = mfe.CSRMatrix.from_host(ro, cols, vals, ctx=ctx)
+    info = mfe.spmv_plan(A)
+    assert info["kernel"] == "sell64-staged-x"
+    runs, _, maxu = staged_plan_model(ro, cols, n)
+    assert (info["runs"], info["max_distinct"]) == (runs, maxu)
+    assert (info["explicit_entries"], info["base_slices"]) == base_slices_model(ro, cols, n)
+    xh = rng.standard_normal(n)
+    x = mfe.Vector.from_numpy(xh, ctx)
+    y = mfe.Vector(n, ctx)
+    mfe.matvec(A, x, y)
+    want = oracle_mod.matvec(Ao, xh)
+    assert bits_equal(y.to_numpy(), want)
+    mfe.matvec_and_dot(A, x, y)
+    assert bits_equal(y.to_numpy(), want)
+    mfe.matvec_fp32(A, x, y)
+    assert bits_equal(y.to_numpy(), oracle_mod.matvec_f32(Ao, xh))
+
+
 @pytest.mark.parametrize("dims", [(10, 10, 10), (40, 33, 29)])
 def test_matvec_fp32_bit_exact(mfe, ctx, oracle_mod, dims):
     """The mixed solve's FP32 SpMV equals the oracle's FP32 restatement bit for bit."""
