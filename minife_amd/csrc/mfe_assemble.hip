@@ -51,6 +51,15 @@ __global__ __launch_bounds__(kBlock) void element_kernel(int Nx, int Ny, int Nz,
   for (int i = 0; i < 8; ++i) Fout[(int64_t)i * ne + e] = F[i];
 }
 
+// The element tables from the kernel arguments into device memory (on the
+// stream, so no host copy is needed).
+__global__ void copy_tables_kernel(HexTables tab, HexTables* __restrict__ out) {
+  const double* src = reinterpret_cast<const double*>(&tab);
+  double* dst = reinterpret_cast<double*>(out);
+  for (int i = (int)threadIdx.x; i < (int)(sizeof(HexTables) / sizeof(double)); i += blockDim.x)
+    dst[i] = src[i];
+}
+
 // One thread per node (row): sum the row's element contributions in ascending
 // element id onto the existing values (sum_into_row adds, it does not store),
 // then store the row's 8..27 entries in ascending column order.  STAGED: the
@@ -385,12 +394,7 @@ static int element_loop(mfe_ctx* ctx, const int gbox[6], const int ebox[6], mfe_
     return hip_fail(ctx, e1, "hipMalloc(element vectors)");
   }
   HexTables* dtab = reinterpret_cast<HexTables*>(Fb + 8 * ne);
-  e1 = hipMemcpy(dtab, &tab, sizeof(HexTables), hipMemcpyHostToDevice);
-  if (e1 != hipSuccess) {
-    hipFree(Kb);
-    hipFree(Fb);
-    return hip_fail(ctx, e1, "element tables H2D");
-  }
+  copy_tables_kernel<<<1, kBlock, 0, ctx->stream>>>(tab, dtab);
   int st = MFE_OK;
   element_kernel<<<(unsigned)cdiv(ne, kBlock), kBlock, 0, ctx->stream>>>(
       Nx, Ny, Nz, 1.0 / Nx, 1.0 / Ny, 1.0 / Nz, dtab, kappa ? kappa->d : nullptr, source, Kb, Fb,
