@@ -284,9 +284,11 @@ __device__ __forceinline__ int2 block_scan2(int a, int b, int* sa, int* sb, int&
 }
 
 // Column range (in columns) up to which a tile's distinct columns are found
-// with a bitmap in LDS instead of a sort: 2^18 bits, the sort's 32 KB.
-constexpr int kBmWords = kXCap;
+// with a bitmap in LDS instead of a sort: 2^19 bits (64 KB, whose first 32 KB
+// hold the sort's keys otherwise), enough for the tiles of a 400^3 mesh.
+constexpr int kBmWords = 16384;
 constexpr int64_t kBmRange = 32 * (int64_t)kBmWords;
+static_assert(kBmWords >= kXCap, "the sort's keys live in the bitmap's storage");
 
 // The slot loop of a plan's pass 1 (shared by both ways of listing the
 // distinct columns): each lane's local column per slot, and per slice whether
@@ -320,9 +322,10 @@ __device__ __forceinline__ void sellx_plan_slots(int n, int nslices, const int* 
   if (lane == 0) saff[slice] = (unsigned char)(all ? 1 : 0);
 }
 
-// Bitmap path of sellx_plan_kernel (see there): bm = kBmWords words of LDS,
-// bit c - cmin set for every column c of the tile.  Thread t owns words
-// [t * kPer, (t + 1) * kPer).
+// Bitmap path of sellx_plan_kernel (see there): bm = the first nw (<=
+// kBmWords) words of LDS, bit c - cmin set for every column c of the tile.
+// Thread t owns words [t * per, (t + 1) * per), per a multiple of 8 (the
+// groups of gpre); words past nw read as 0.
 __device__ void sellx_plan_bitmap(int n, int nslices, const int* __restrict__ ro,
                                   const int* __restrict__ cols, const int* __restrict__ soff,
                                   const unsigned char* __restrict__ slen, int pass,
@@ -331,21 +334,22 @@ __device__ void sellx_plan_bitmap(int n, int nslices, const int* __restrict__ ro
                                   unsigned short* __restrict__ lcols,
                                   unsigned char* __restrict__ saff,
                                   unsigned short* __restrict__ sbase, unsigned* bm, int* gpre,
-                                  int* sa, int* sb, int s0, int cnt, int cmin) {
-  constexpr int kPer = kBmWords / kBlock;  // 32 words per thread = 4 groups of 8
+                                  int* sa, int* sb, int s0, int cnt, int cmin, int nw) {
+  const int per = ((nw + kBlock - 1) / kBlock + 7) & ~7;
   const int t = (int)threadIdx.x, tile = (int)blockIdx.x;
-  for (int i = t; i < kBmWords; i += kBlock) bm[i] = 0u;
+  for (int i = t; i < nw; i += kBlock) bm[i] = 0u;
   __syncthreads();
   for (int i = t; i < cnt; i += kBlock) {
     const int d = cols[s0 + i] - cmin;
     atomicOr(&bm[d >> 5], 1u << (d & 31));
   }
   __syncthreads();
-  const int w0 = t * kPer;
-  unsigned prev_msb = w0 > 0 ? bm[w0 - 1] >> 31 : 0u;
+  const int w0 = t * per;
+  auto word = [&](int w) { return w < nw ? bm[w] : 0u; };
+  unsigned prev_msb = w0 > 0 ? word(w0 - 1) >> 31 : 0u;
   int nu = 0, nr = 0;
-  for (int q = 0; q < kPer; ++q) {
-    const unsigned wd = bm[w0 + q];
+  for (int q = 0; q < per; ++q) {
+    const unsigned wd = word(w0 + q);
     const unsigned st = wd & ~((wd << 1) | prev_msb);  // set bits whose predecessor is clear
     nu += __popc(wd);
     nr += __popc(st);
@@ -360,10 +364,10 @@ __device__ void sellx_plan_bitmap(int n, int nslices, const int* __restrict__ ro
   // runs in increasing column order, and the set bits before each group of 8 words
   int u = base.x, r = base.y;
   const int rb = roff[tile];
-  prev_msb = w0 > 0 ? bm[w0 - 1] >> 31 : 0u;
-  for (int q = 0; q < kPer; ++q) {
-    if ((q & 7) == 0) gpre[(w0 + q) >> 3] = u;
-    const unsigned wd = bm[w0 + q];
+  prev_msb = w0 > 0 ? word(w0 - 1) >> 31 : 0u;
+  for (int q = 0; q < per; ++q) {
+    if ((q & 7) == 0 && w0 + q < nw) gpre[(w0 + q) >> 3] = u;
+    const unsigned wd = word(w0 + q);
     unsigned st = wd & ~((wd << 1) | prev_msb);
     while (st) {
       const int bit = __ffs(st) - 1;
@@ -402,7 +406,8 @@ __global__ __launch_bounds__(kBlock) void sellx_plan_kernel(
     int2* __restrict__ counts, const int* __restrict__ roff, int* __restrict__ rcol,
     int* __restrict__ rpos, unsigned short* __restrict__ lcols, unsigned char* __restrict__ saff,
     unsigned short* __restrict__ sbase) {
-  __shared__ int a[kXCap];
+  __shared__ unsigned bmw[kBmWords];  // the bitmap, or (first kXCap) the sort's keys
+  int* const a = reinterpret_cast<int*>(bmw);
   __shared__ int sa[kBlock], sb[kBlock];
   __shared__ int gpre[kBmWords / 8];  // bitmap path: set bits before each group of 8 words
   constexpr int kPer = kXCap / kBlock;
@@ -435,8 +440,8 @@ __global__ __launch_bounds__(kBlock) void sellx_plan_kernel(
   __syncthreads();
   if (cnt == 0 || (int64_t)cmax - cmin + 1 <= kBmRange) {
     sellx_plan_bitmap(n, nslices, ro, cols, soff, slen, pass, counts, roff, rcol, rpos, lcols,
-                      saff, sbase, reinterpret_cast<unsigned*>(a), gpre, sa, sb, s0, cnt,
-                      cnt == 0 ? 0 : cmin);
+                      saff, sbase, bmw, gpre, sa, sb, s0, cnt, cnt == 0 ? 0 : cmin,
+                      cnt == 0 ? 0 : (int)(((int64_t)cmax - cmin + 32) >> 5));
     return;
   }
   for (int i = t; i < kXCap; i += kBlock) a[i] = i < cnt ? cols[s0 + i] : INT_MAX;

@@ -487,22 +487,24 @@ def test_dirichlet_on_uploaded_matrices(mfe, ctx, oracle_mod, maxlen):
 
 def test_staged_x_plan_bitmap_and_sort_tiles(mfe, ctx, oracle_mod):
     """The plan lists a tile's distinct columns with an LDS bitmap when they
-    span at most 2^18 columns and by sorting otherwise.  400,000 rows of a
-    3-point band, where the even tiles of the first and last 100,000 rows also
-    reach a column 300,000 away (a span above 2^18: sorted) and every other
-    tile stays within its band (bitmap): the plan's counts match the numpy
-    model, and SpMV, SpMV+dot and the FP32 SpMV equal the oracle bit for bit."""
+    span at most 2^19 columns and by sorting otherwise.  700,000 rows of a
+    3-point band; tiles 0, 3, 6, ... also reach a column 600,000 away (a span
+    above 2^19: sorted), tiles 1, 4, 7, ... one 300,000 away (a wide bitmap)
+    where the matrix has such columns, the rest stay within the band: the
+    plan's counts match the numpy model, and SpMV, SpMV+dot and the FP32 SpMV
+    equal the oracle bit for bit."""
     say("case staged x bitmap and sort tiles")
-    n, far = 400_000, 300_000
+    n = 700_000
     rng = np.random.default_rng(23)
     parts = []
     for r in range(n):
         c = [v for v in (r - 1, r, r + 1) if 0 <= v < n]
-        if (r // 256) % 2 == 0:
-            if r + far < n:
-                c.append(r + far)
-            elif r - far >= 0:
-                c.insert(0, r - far)
+        kind = (r // 256) % 3
+        far = 600_000 if kind == 0 else 300_000 if kind == 1 else 0
+        if far and r + far < n:
+            c.append(r + far)
+        elif far and r - far >= 0:
+            c.insert(0, r - far)
         parts.append(c)
     ro = np.concatenate([[0], np.cumsum([len(c) for c in parts])]).astype(np.int32)
     cols = np.fromiter((v for c in parts for v in c), dtype=np.int32, count=int(ro[-1]))
