@@ -10,6 +10,7 @@
 // element kernel evaluates Hex8::diffusionMatrix_symm / sourceVector with the
 // reference's operation order; -ffp-contract=off keeps every product rounded.
 #include <cmath>
+#include <cstring>
 
 #include "mfe_common.hpp"
 #include "mfe_hex8.hpp"
@@ -217,6 +218,33 @@ __global__ __launch_bounds__(kBlock) void structure_kernel(int gx, int gy, int g
   for (int i = (int)threadIdx.x; i < cnt; i += kBlock) cols[s0 + i] = sc[i];
 }
 
+// Sets *bad when a row's offsets or columns differ from structure_kernel's.
+__global__ __launch_bounds__(kBlock) void structure_check_kernel(int gx, int gy, int gz,
+                                                                 const int* __restrict__ ro,
+                                                                 const int* __restrict__ cols,
+                                                                 unsigned* bad) {
+  const int64_t n = (int64_t)gx * gy * gz;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const int ix = (int)(id % gx);
+  const int iy = (int)((id / gx) % gy);
+  const int iz = (int)(id / ((int64_t)gx * gy));
+  const int64_t Sx = 3 * (int64_t)gx - 2, Sy = 3 * (int64_t)gy - 2;
+  const int64_t cz = axis_cnt(iz, gz), cy = axis_cnt(iy, gy);
+  const int64_t off = axis_pre(iz) * Sy * Sx + cz * axis_pre(iy) * Sx + cz * cy * axis_pre(ix);
+  const int64_t len = cz * cy * axis_cnt(ix, gx);
+  bool ok = ro[id] == off && ro[id + 1] == off + len;
+  int64_t pos = off;
+  for (int sz = -1; sz <= 1 && ok; ++sz)
+    for (int sy = -1; sy <= 1 && ok; ++sy)
+      for (int sx = -1; sx <= 1 && ok; ++sx) {
+        const int x = ix + sx, y = iy + sy, z = iz + sz;
+        if (x < 0 || y < 0 || z < 0 || x >= gx || y >= gy || z >= gz) continue;
+        ok = cols[pos++] == (int)(x + (int64_t)gx * y + (int64_t)gx * gy * z);
+      }
+  if (!ok) atomicOr(bad, 1u);
+}
+
 __global__ void bc_flag_kernel(int64_t nbc, const int* __restrict__ bc, int64_t n,
                                unsigned char* __restrict__ flags) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -358,6 +386,9 @@ int mfe_generate_matrix_structure(mfe_ctx* ctx, const int gbox[6], const int lbo
   A->has_local_indices = false;
   A->num_cols = 0;
   csr_structure_changed(A);
+  A->box_elems[0] = gbox[1];
+  A->box_elems[1] = gbox[3];
+  A->box_elems[2] = gbox[5];
   return plan_spmv(ctx, A);
 }
 
@@ -381,6 +412,25 @@ static int element_loop(mfe_ctx* ctx, const int gbox[6], const int ebox[6], mfe_
   const int64_t ne = (int64_t)Nx * Ny * Nz;
   if (A->nrows != n || A->nnz != nnz || b->n != n)
     return fail(ctx, MFE_BAD_ARG, "A/b do not hold this box's structure");
+  if (A->box_elems[0] != Nx || A->box_elems[1] != Ny || A->box_elems[2] != Nz) {
+    // a structure from elsewhere (an upload): accepted if it is the box's
+    unsigned* bad = reinterpret_cast<unsigned*>(ctx->d_scalars);
+    MFE_CHECK(ctx, hipMemsetAsync(bad, 0, sizeof(unsigned), ctx->stream));
+    structure_check_kernel<<<(unsigned)cdiv(n, kBlock), kBlock, 0, ctx->stream>>>(
+        Nx + 1, Ny + 1, Nz + 1, A->row_offsets, A->cols, bad);
+    MFE_LAUNCHED(ctx, "structure_check_kernel");
+    MFE_CHECK(ctx, hipMemcpyAsync(ctx->h_scalars, bad, sizeof(unsigned), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    unsigned hbad = 0;
+    std::memcpy(&hbad, ctx->h_scalars, sizeof(hbad));
+    if (hbad)
+      return fail(ctx, MFE_BAD_ARG,
+                  "A's structure is not the box's (generate_matrix_structure's rows and columns)");
+    A->box_elems[0] = Nx;
+    A->box_elems[1] = Ny;
+    A->box_elems[2] = Nz;
+  }
   if (kappa && kappa->n != ne) return fail(ctx, MFE_BAD_ARG, "kappa must have one value per element");
   HexTables tab;
   host_tables(tab);

@@ -56,6 +56,10 @@ struct mfe_csr {
   int64_t num_cols = 0;
   // SpMV plan: largest nnz of any kSpmvRows-row block (-1 = not computed).
   int64_t max_block_nnz = -1;
+  // Elements per side of the box whose stencil generate_matrix_structure wrote
+  // (0 after any other change of structure): the element loop writes each
+  // row's entries at the positions that structure gives them.
+  int box_elems[3] = {0, 0, 0};
   // SELL-64 copy of the matrix for the SpMV (sliced ELLPACK: slices of 64
   // rows, one row per lane, slot j of a slice stored as 64 consecutive
   // entries), derived lazily from the CSR arrays and rebuilt after changes.
@@ -124,6 +128,7 @@ inline void csr_values_changed(mfe_csr* A) {
 }
 inline void csr_structure_changed(mfe_csr* A) {
   csr_values_changed(A);
+  A->box_elems[0] = A->box_elems[1] = A->box_elems[2] = 0;
   A->plan_gen = next_plan_gen();
   A->max_block_nnz = -1;
   A->sell_struct_valid = false;

@@ -113,6 +113,30 @@ def test_element_loop_accumulates_like_the_reference(mfe, ctx, oracle_mod):
     assert bits_equal(b.to_numpy(), bo)
 
 
+def test_element_loop_on_uploaded_structures(mfe, ctx, oracle_mod):
+    """The element loop writes each row's entries where the box stencil puts
+    them: an uploaded copy of the box's structure is checked and assembled
+    like a generated one (bit-exact vs the oracle); a structure of the same
+    size with two columns swapped is refused with MFE_BAD_ARG."""
+    say("case element loop on uploaded structures")
+    dims = (6, 5, 4)
+    mesh = mfe.simple_mesh_description([0, 6, 0, 5, 0, 4])
+    S = oracle_mod.generate_matrix_structure(*dims)
+    A = mfe.CSRMatrix.from_host(S.row_offsets, S.cols, np.zeros(S.nnz), ctx=ctx)
+    b = mfe.Vector(A.nrows, ctx)
+    mfe.assemble_FE_data(mesh, A, b)
+    Ao = oracle_mod.generate_matrix_structure(*dims)
+    bo = oracle_mod.assemble_FE_data(*dims, Ao)
+    assert bits_equal(A.to_host()["vals"], Ao.vals)
+    assert bits_equal(b.to_numpy(), bo)
+    cols = S.cols.copy()
+    cols[[40, 41]] = cols[[41, 40]]
+    B = mfe.CSRMatrix.from_host(S.row_offsets, cols, np.zeros(S.nnz), ctx=ctx)
+    with pytest.raises(mfe.MfeError) as e:
+        mfe.assemble_FE_data(mesh, B, mfe.Vector(B.nrows, ctx))
+    assert e.value.status == mfe.MFE_BAD_ARG
+
+
 def test_kappa_assembly_matches_oracle(mfe, ctx, oracle_mod):
     say("case kappa assembly 6x5x4")
     dims = (6, 5, 4)
