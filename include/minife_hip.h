@@ -104,12 +104,13 @@ int mfe_generate_matrix_structure(mfe_ctx* ctx, const int global_box[6],
 
 /* perform_element_loop(mesh, elem_box, A, b, params),
  * perform_element_loop.hpp:40-104: hex8 diffusion matrix and source vector per
- * element (Hex8.hpp:259-410), summed into A and b.  Each entry receives its
- * elements' contributions in ascending element id, from 0.0, so the result
- * is bit-identical to the reference.  A must hold the structure of
- * mfe_generate_matrix_structure; values and b are overwritten.  kappa
- * (nullable, one double per element) is the per-element conductivity k of
- * Hex8.hpp:273 (reference: 1.0). */
+ * element (Hex8.hpp:259-410), added onto A's values and b (sum_into_row
+ * adds).  Each entry receives its elements' contributions in ascending
+ * element id, so the result is bit-identical to the reference.  A must hold
+ * the box's structure: the one mfe_generate_matrix_structure writes (a copy
+ * uploaded with mfe_csr_upload is verified on the device first); otherwise
+ * MFE_BAD_ARG.  kappa (nullable, one double per element) is the per-element
+ * conductivity k of Hex8.hpp:273 (reference: 1.0). */
 int mfe_perform_element_loop(mfe_ctx* ctx, const int global_box[6], const int elem_box[6],
                              mfe_csr* A, mfe_vec* b, const mfe_vec* kappa);
 /* assemble_FE_data(mesh, A, b, params), assemble_FE_data.hpp:38-73. */
