@@ -63,14 +63,16 @@ __global__ void copy_tables_kernel(HexTables tab, HexTables* __restrict__ out) {
 
 // One thread per node (row): sum the row's element contributions in ascending
 // element id onto the existing values (sum_into_row adds, it does not store),
-// then store the row's 8..27 entries in ascending column order.  STAGED: the
-// workgroup's 256 rows of values (at most 27 * 256 entries, contiguous in
-// vals) are read into LDS and written back with coalesced accesses; each
-// thread's row is then read and written in LDS (a row's 27 entries are 216
-// bytes apart from the next row's, so direct per-thread accesses touch 64
-// cache lines per instruction).  Same arithmetic either way.
-template <bool STAGED>
-__global__ __launch_bounds__(kBlock) void gather_kernel(int Nx, int Ny, int Nz,
+// then store the row's 8..27 entries in ascending column order.  The
+// workgroup's rows of values (one wave of 64 rows, at most 27 * 64 entries,
+// contiguous in vals: A holds the box's structure) are read into LDS and
+// written back with coalesced accesses; each thread's row is read and
+// written in LDS (a row's 27 entries are 216 bytes apart from the next row's,
+// so direct per-thread accesses touch 64 cache lines per instruction: 9.6 ms
+// at 300^3).  One wave per workgroup keeps 14 KB of LDS each, so 11 waves fit
+// a CU (FE 10.7 ms against 12.1 ms with 256-row workgroups, 2 per CU).
+constexpr int kGatherRows = kWave;
+__global__ __launch_bounds__(kGatherRows) void gather_kernel(int Nx, int Ny, int Nz,
                                                         const int* __restrict__ ro,
                                                         const double* __restrict__ K,
                                                         const double* __restrict__ F, int64_t ne,
@@ -79,19 +81,15 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(int Nx, int Ny, int Nz,
   const int gx = Nx + 1, gy = Ny + 1;
   const int64_t n = (int64_t)gx * gy * (Nz + 1);
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  __shared__ double sv[STAGED ? kSpmvMaxNnz : 1];
-  int64_t s0 = 0;
-  int cnt = 0;
-  if (STAGED) {
-    const int64_t r0 = (int64_t)blockIdx.x * blockDim.x;
-    const int64_t r1 = r0 + kBlock < n ? r0 + kBlock : n;
-    s0 = ro[r0];
-    cnt = (int)(ro[r1] - s0);
-    for (int i = (int)threadIdx.x; i < cnt; i += kBlock) sv[i] = vals[s0 + i];
-    __syncthreads();
-  }
+  __shared__ double sv[27 * kGatherRows];
+  const int64_t r0 = (int64_t)blockIdx.x * kGatherRows;
+  const int64_t r1 = r0 + kGatherRows < n ? r0 + kGatherRows : n;
+  const int64_t s0 = ro[r0];
+  const int cnt = (int)(ro[r1] - s0);
+  for (int i = (int)threadIdx.x; i < cnt; i += kGatherRows) sv[i] = vals[s0 + i];
+  __syncthreads();
   if (id < n) {
-    double* const rv = STAGED ? sv : vals;  // the row's values: LDS (tile-relative) or HBM
+    double* const rv = sv;  // the row's values, tile-relative
     const int ix = (int)(id % gx);
     const int iy = (int)((id / gx) % gy);
     const int iz = (int)(id / ((int64_t)gx * gy));
@@ -151,10 +149,8 @@ __global__ __launch_bounds__(kBlock) void gather_kernel(int Nx, int Ny, int Nz,
         }
     b[id] = accb;
   }
-  if (STAGED) {
-    __syncthreads();
-    for (int i = (int)threadIdx.x; i < cnt; i += kBlock) vals[s0 + i] = sv[i];
-  }
+  __syncthreads();
+  for (int i = (int)threadIdx.x; i < cnt; i += kGatherRows) vals[s0 + i] = sv[i];
 }
 
 // generate_matrix_structure + MatrixInitOp: one thread per node.  The row
@@ -451,14 +447,9 @@ static int element_loop(mfe_ctx* ctx, const int gbox[6], const int ebox[6], mfe_
       ne);
   hipError_t le = hipGetLastError();
   if (le == hipSuccess) {
-    // the box structure has at most 27 entries per row; a matrix whose tiles
-    // hold more (possible only if it was replaced by an upload) is not staged
-    if (A->max_block_nnz >= 0 && A->max_block_nnz <= kSpmvMaxNnz)
-      gather_kernel<true><<<(unsigned)cdiv(n, kBlock), kBlock, 0, ctx->stream>>>(
-          Nx, Ny, Nz, A->row_offsets, Kb, Fb, ne, A->vals, b->d);
-    else
-      gather_kernel<false><<<(unsigned)cdiv(n, kBlock), kBlock, 0, ctx->stream>>>(
-          Nx, Ny, Nz, A->row_offsets, Kb, Fb, ne, A->vals, b->d);
+    // the box structure (checked above) has at most 27 entries per row
+    gather_kernel<<<(unsigned)cdiv(n, kGatherRows), kGatherRows, 0, ctx->stream>>>(
+        Nx, Ny, Nz, A->row_offsets, Kb, Fb, ne, A->vals, b->d);
     le = hipGetLastError();
   }
   csr_values_changed(A);

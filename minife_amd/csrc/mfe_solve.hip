@@ -189,54 +189,34 @@ __global__ __launch_bounds__(kBlock) void sell_len_kernel(int n, int nslices,
   if (row % kSlice == 0) width[slice] = m;
 }
 
-// Scatter the CSR rows into the slices: columns when scols is non-null,
-// values as V (double, or rounded to float) when svals is non-null.  Padding: value 0, column =
-// the row itself (rows past n use n-1), so every gather stays in bounds.
-template <typename V>
-__global__ __launch_bounds__(kBlock) void sell_fill_kernel(
-    int n, int nslices, const int* __restrict__ soff, const unsigned char* __restrict__ slen,
-    const int* __restrict__ ro, const int* __restrict__ cols, const double* __restrict__ vals,
-    int* __restrict__ scols, V* __restrict__ svals) {
-  const int row = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int slice = row / kSlice;
-  if (slice >= nslices) return;
-  const int o = soff[slice], w = soff[slice + 1] - o;
-  const int len = slen[row];
-  const int start = row < n ? ro[row] : 0;
-  const int pad = row < n ? row : n - 1;
-  for (int j = 0; j < w; ++j) {
-    const int64_t at = sell_pos((int64_t)o * kSlice, j, w, row % kSlice);
-    const bool in = j < len;
-    if (scols) scols[at] = in ? cols[start + j] : pad;
-    if (svals) svals[at] = in ? (V)vals[start + j] : V(0);
-  }
-}
-
-// sell_fill_kernel for matrices whose 256-row tiles (the workgroup's four
-// slices) hold at most kSpmvMaxNnz entries: the tile's CSR columns or values
-// (contiguous) are read into LDS with coalesced loads first -- a row's entries
+// Scatter the CSR rows into the slices, one workgroup (one wave) per slice:
+// the slice's CSR columns or values (S = int or double; at most 27 per row,
+// contiguous) are read into LDS with coalesced loads first -- a row's entries
 // are a whole row apart from the next lane's, so direct per-thread reads touch
-// 64 cache lines per instruction.  S = int (columns) or double (values).
+// 64 cache lines per instruction -- then each lane writes its row's slots.
+// D = int (columns), double or float (values, rounded).  Padding: value 0,
+// column = the row itself (rows past n use n-1), so every gather stays in
+// bounds.
 template <typename S, typename D>
-__global__ __launch_bounds__(kBlock) void sell_fill_tile_kernel(
+__global__ __launch_bounds__(kSlice) void sell_fill_kernel(
     int n, int nslices, const int* __restrict__ soff, const unsigned char* __restrict__ slen,
     const int* __restrict__ ro, const S* __restrict__ src, D* __restrict__ dst, bool columns) {
-  __shared__ S ss[kSpmvMaxNnz];
-  const int t = (int)threadIdx.x;
-  const int r0 = (int)blockIdx.x * kBlock;
-  const int rl = min(r0, n), rh = min(r0 + kBlock, n);
-  const int s0 = ro[rl], cnt = ro[rh] - s0;
-  for (int i = t; i < cnt; i += kBlock) ss[i] = src[s0 + i];
-  __syncthreads();
-  const int row = r0 + t;
-  const int slice = row / kSlice;
+  __shared__ S ss[kSellMaxWidth * kSlice];
+  const int slice = (int)blockIdx.x;
   if (slice >= nslices) return;
+  const int lane = (int)threadIdx.x;
+  const int r0 = slice * kSlice;
+  const int rl = min(r0, n), rh = min(r0 + kSlice, n);
+  const int s0 = ro[rl], cnt = ro[rh] - s0;
+  for (int i = lane; i < cnt; i += kSlice) ss[i] = src[s0 + i];
+  __syncthreads();
+  const int row = r0 + lane;
   const int o = soff[slice], w = soff[slice + 1] - o;
   const int len = slen[row];
   const int start = row < n ? ro[row] - s0 : 0;
   const int pad = row < n ? row : n - 1;
   for (int j = 0; j < w; ++j) {
-    const int64_t at = sell_pos((int64_t)o * kSlice, j, w, row % kSlice);
+    const int64_t at = sell_pos((int64_t)o * kSlice, j, w, lane);
     const bool in = j < len;
     if (columns) dst[at] = in ? (D)ss[start + j] : (D)pad;
     else dst[at] = in ? (D)ss[start + j] : D(0);
@@ -810,14 +790,9 @@ static int ensure_plan(mfe_ctx* ctx, const mfe_csr* Ac, bool need64, bool need32
         MFE_CHECK(ctx, hipMemsetAsync(A->sell_cols + A->sell_slots, 0, sizeof(int) * kSellTail, s));
         MFE_CHECK(ctx, hipMemcpyAsync(A->sell_off, off.data(), sizeof(int) * (size_t)(ns + 1),
                                       hipMemcpyHostToDevice, s));
-        if (A->max_block_nnz >= 0 && A->max_block_nnz <= kSpmvMaxNnz)
-          sell_fill_tile_kernel<int, int><<<(unsigned)cdiv(ns * kSlice, kBlock), kBlock, 0, s>>>(
-              (int)n, (int)ns, A->sell_off, A->sell_len, A->row_offsets, A->cols, A->sell_cols,
-              true);
-        else
-          sell_fill_kernel<double><<<(unsigned)cdiv(ns * kSlice, kBlock), kBlock, 0, s>>>(
-              (int)n, (int)ns, A->sell_off, A->sell_len, A->row_offsets, A->cols, A->vals,
-              A->sell_cols, nullptr);
+        sell_fill_kernel<int, int><<<(unsigned)ns, kSlice, 0, s>>>(
+            (int)n, (int)ns, A->sell_off, A->sell_len, A->row_offsets, A->cols, A->sell_cols,
+            true);
         MFE_LAUNCHED(ctx, "sell_fill_kernel (columns)");
       }
     }
@@ -834,8 +809,7 @@ static int ensure_plan(mfe_ctx* ctx, const mfe_csr* Ac, bool need64, bool need32
     if ((st = build_sellx(ctx, A))) return st;
   }
   const bool sell = sell_format && A->sell_ok && n > 0;
-  const unsigned fg = (unsigned)cdiv(A->sell_nslices * kSlice, kBlock);
-  const bool tiles_fit = A->max_block_nnz >= 0 && A->max_block_nnz <= kSpmvMaxNnz;
+  const unsigned fg = (unsigned)A->sell_nslices;  // one wave per slice
   if (sell && need64 && !A->sell_vals_valid) {
     if (!A->sell_vals) {
       if (!try_alloc(&A->sell_vals, sizeof(double) * (size_t)(A->sell_slots + kSellTail))) {
@@ -844,14 +818,9 @@ static int ensure_plan(mfe_ctx* ctx, const mfe_csr* Ac, bool need64, bool need32
       }
       MFE_CHECK(ctx, hipMemsetAsync(A->sell_vals + A->sell_slots, 0, sizeof(double) * kSellTail, s));
     }
-    if (tiles_fit)
-      sell_fill_tile_kernel<double, double><<<fg, kBlock, 0, s>>>(
-          (int)n, (int)A->sell_nslices, A->sell_off, A->sell_len, A->row_offsets, A->vals,
-          A->sell_vals, false);
-    else
-      sell_fill_kernel<double><<<fg, kBlock, 0, s>>>((int)n, (int)A->sell_nslices, A->sell_off,
-                                                     A->sell_len, A->row_offsets, A->cols,
-                                                     A->vals, nullptr, A->sell_vals);
+    sell_fill_kernel<double, double><<<fg, kSlice, 0, s>>>(
+        (int)n, (int)A->sell_nslices, A->sell_off, A->sell_len, A->row_offsets, A->vals,
+        A->sell_vals, false);
     MFE_LAUNCHED(ctx, "sell_fill_kernel<double>");
     A->sell_vals_valid = true;
   }
@@ -863,14 +832,9 @@ static int ensure_plan(mfe_ctx* ctx, const mfe_csr* Ac, bool need64, bool need32
       }
       MFE_CHECK(ctx, hipMemsetAsync(A->sell_vals32 + A->sell_slots, 0, sizeof(float) * kSellTail, s));
     }
-    if (tiles_fit)
-      sell_fill_tile_kernel<double, float><<<fg, kBlock, 0, s>>>(
-          (int)n, (int)A->sell_nslices, A->sell_off, A->sell_len, A->row_offsets, A->vals,
-          A->sell_vals32, false);
-    else
-      sell_fill_kernel<float><<<fg, kBlock, 0, s>>>((int)n, (int)A->sell_nslices, A->sell_off,
-                                                    A->sell_len, A->row_offsets, A->cols,
-                                                    A->vals, nullptr, A->sell_vals32);
+    sell_fill_kernel<double, float><<<fg, kSlice, 0, s>>>(
+        (int)n, (int)A->sell_nslices, A->sell_off, A->sell_len, A->row_offsets, A->vals,
+        A->sell_vals32, false);
     MFE_LAUNCHED(ctx, "sell_fill_kernel<float>");
     A->sell_vals32_valid = true;
   }
