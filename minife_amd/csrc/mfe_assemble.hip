@@ -311,6 +311,44 @@ __global__ __launch_bounds__(kBlock) void dirichlet_tile_kernel(
   b[row] -= sum * value;
 }
 
+// On a structurally symmetric matrix (the box stencil) the rows that hold a
+// BC column are the columns of the BC rows: mark them (every writer stores
+// 1, so overlapping marks need no atomics).
+__global__ void bc_touch_kernel(int64_t nbc, const int* __restrict__ bc, int64_t n,
+                                const int* __restrict__ ro, const int* __restrict__ cols,
+                                unsigned char* __restrict__ touch) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nbc) return;
+  const int r = bc[k];
+  if (r < 0 || r >= n) return;  // ignored, as by bc_flag_kernel
+  for (int j = ro[r]; j < ro[r + 1]; ++j) touch[cols[j]] = 1;
+}
+
+// dirichlet_kernel for the marked rows only: any other row has no BC column,
+// and dirichlet_kernel would leave it as it is (its sum stays 0 and b[row]
+// - 0 * value is b[row]).
+__global__ __launch_bounds__(kBlock) void dirichlet_touched_kernel(
+    int64_t n, const int* __restrict__ ro, const int* __restrict__ cols,
+    double* __restrict__ vals, double* __restrict__ b, const unsigned char* __restrict__ flags,
+    const unsigned char* __restrict__ touch, double value) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n || !(touch[row] | flags[row])) return;
+  const int j0 = ro[row], j1 = ro[row + 1];
+  if (flags[row]) {
+    b[row] = value;
+    for (int j = j0; j < j1; ++j) vals[j] = (cols[j] == row) ? 1.0 : 0.0;
+    return;
+  }
+  double sum = 0;
+  for (int j = j0; j < j1; ++j) {
+    if (flags[cols[j]]) {
+      sum += vals[j];
+      vals[j] = 0;
+    }
+  }
+  b[row] -= sum * value;
+}
+
 // grad[e] = -lambda_e^T K_e(1) utilde_e, one thread per element.
 __global__ __launch_bounds__(kBlock) void element_gradient_kernel(
     int Nx, int Ny, int Nz, double hx, double hy, double hz, HexTables tab,
@@ -492,25 +530,36 @@ int mfe_impose_dirichlet(mfe_ctx* ctx, double value, mfe_csr* A, mfe_vec* b, int
   unsigned char* flags = nullptr;
   hipError_t e = hipMalloc(&d_bc, sizeof(int) * (size_t)(nbc > 0 ? nbc : 1));
   if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc(bc rows)");
-  e = hipMalloc(&flags, (size_t)n);
+  // flags[0, n): BC rows; flags[n, 2n): rows that may hold a BC column
+  e = hipMalloc(&flags, 2 * (size_t)n);
   if (e != hipSuccess) {
     hipFree(d_bc);
     return hip_fail(ctx, e, "hipMalloc(bc flags)");
   }
+  unsigned char* const touch = flags + n;
+  // the box stencil is structurally symmetric: only the BC rows' neighbours
+  // can hold BC columns
+  const bool box = A->box_elems[0] > 0;
   int st = MFE_OK;
   do {
     if (nbc > 0) {
       e = hipMemcpyAsync(d_bc, bc_rows, sizeof(int) * nbc, hipMemcpyHostToDevice, ctx->stream);
       if (e != hipSuccess) { st = hip_fail(ctx, e, "bc rows H2D"); break; }
     }
-    e = hipMemsetAsync(flags, 0, (size_t)n, ctx->stream);
+    e = hipMemsetAsync(flags, 0, 2 * (size_t)n, ctx->stream);
     if (e != hipSuccess) { st = hip_fail(ctx, e, "bc flags memset"); break; }
     if (nbc > 0) {
       bc_flag_kernel<<<(unsigned)cdiv(nbc, kBlock), kBlock, 0, ctx->stream>>>(nbc, d_bc, n, flags);
       if ((e = hipGetLastError()) != hipSuccess) { st = hip_fail(ctx, e, "bc_flag_kernel"); break; }
     }
     csr_values_changed(A);
-    if (A->max_block_nnz >= 0 && A->max_block_nnz <= kSpmvMaxNnz)
+    if (box) {
+      if (nbc > 0)
+        bc_touch_kernel<<<(unsigned)cdiv(nbc, kBlock), kBlock, 0, ctx->stream>>>(
+            nbc, d_bc, n, A->row_offsets, A->cols, touch);
+      dirichlet_touched_kernel<<<(unsigned)cdiv(n, kBlock), kBlock, 0, ctx->stream>>>(
+          n, A->row_offsets, A->cols, A->vals, b->d, flags, touch, value);
+    } else if (A->max_block_nnz >= 0 && A->max_block_nnz <= kSpmvMaxNnz)
       dirichlet_tile_kernel<<<(unsigned)cdiv(n, kBlock), kBlock, 0, ctx->stream>>>(
           n, A->row_offsets, A->cols, A->vals, b->d, flags, value);
     else
