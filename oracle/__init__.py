@@ -1,9 +1,10 @@
 """ctypes binding of the C restatement of miniFE ref/ (liboracle_minife.so).
 
 TEST INFRASTRUCTURE ONLY: the parity oracle for the HIP library and the
-"port" CPU baseline.  Only tests/, ``__graft_entry__.smoke`` and bench.py's
-CPU-baseline leg import this module, always as the checker, never as the
-thing measured or shipped.  The product (``minife_amd``) never imports it.
+"port" CPU baseline.  Only tests/, the smoke test and bench.py's CPU-baseline
+leg import this module, always as the reference results are compared with,
+never as the thing measured or shipped.  The product (``minife_amd``) never
+imports it.
 
 Functions mirror the reference operators (see minife_oracle.h for the
 file:line each restates).  Arrays are numpy; indices int32, values float64.
