@@ -242,8 +242,8 @@ def main():
                     help="run just this extra workload and print its JSON (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(every GPU box here has one GPU); run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (the sharding "
+              "design is described, not built); run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import minife_amd as mfe
