@@ -27,6 +27,53 @@ __global__ void tile_nnz_max_kernel(int64_t n, const int* __restrict__ ro, int64
   atomicMax(out, (unsigned long long)(c < 0 ? 0 : c));
 }
 
+// Uploads from pageable host memory start the runtime's copy machinery,
+// which costs several ms the first time in a process; small host lists the
+// kernels only read once (impose_dirichlet's BC rows) are read by the kernels
+// straight from this pinned buffer instead.
+int stage_buffer(mfe_ctx* ctx, void** out) {
+  if (!ctx->h_stage) MFE_CHECK(ctx, hipHostMalloc(&ctx->h_stage, kStageBytes, hipHostMallocDefault));
+  *out = ctx->h_stage;
+  return MFE_OK;
+}
+
+// dst[i] = src[i] for i < nb, four bytes per thread.
+__global__ void copy_bytes_kernel(int64_t nb, const unsigned char* __restrict__ src,
+                                  unsigned char* __restrict__ dst) {
+  const int64_t i0 = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (i0 + k < nb) dst[i0 + k] = src[i0 + k];
+}
+
+static int staged_copy(mfe_ctx* ctx, void* dst, const void* src, size_t bytes, bool up) {
+  if (bytes == 0) return MFE_OK;
+  void* stage = nullptr;
+  int st = stage_buffer(ctx, &stage);
+  if (st) return st;
+  const unsigned char* s = static_cast<const unsigned char*>(src);
+  unsigned char* d = static_cast<unsigned char*>(dst);
+  unsigned char* pin = static_cast<unsigned char*>(stage);
+  for (size_t off = 0; off < bytes; off += kStageBytes) {
+    const size_t len = bytes - off < kStageBytes ? bytes - off : kStageBytes;
+    if (up) std::memcpy(pin, s + off, len);
+    copy_bytes_kernel<<<(unsigned)cdiv((int64_t)cdiv((int64_t)len, 4), kBlock), kBlock, 0,
+                        ctx->stream>>>((int64_t)len, up ? pin : s + off, up ? d + off : pin);
+    MFE_LAUNCHED(ctx, "copy_bytes_kernel");
+    MFE_CHECK(ctx, hipStreamSynchronize(ctx->stream));  // the buffer is reused next
+    if (!up) std::memcpy(d + off, pin, len);
+  }
+  return MFE_OK;
+}
+
+int upload_bytes(mfe_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  return staged_copy(ctx, dst, src, bytes, true);
+}
+
+int download_bytes(mfe_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  return staged_copy(ctx, dst, src, bytes, false);
+}
+
 int plan_spmv(mfe_ctx* ctx, mfe_csr* A) {
   if (A->nrows == 0) {
     A->max_block_nnz = 0;
@@ -92,6 +139,7 @@ void mfe_ctx_destroy(mfe_ctx* c) {
   if (c->d_partials) hipFree(c->d_partials);
   if (c->d_scalars) hipFree(c->d_scalars);
   if (c->h_scalars) hipHostFree(c->h_scalars);
+  if (c->h_stage) hipHostFree(c->h_stage);
   if (c->t_start) hipEventDestroy(c->t_start);
   if (c->t_stop) hipEventDestroy(c->t_stop);
   if (c->stream) hipStreamDestroy(c->stream);

@@ -526,40 +526,45 @@ int mfe_impose_dirichlet(mfe_ctx* ctx, double value, mfe_csr* A, mfe_vec* b, int
   MFE_CHECK(ctx, hipSetDevice(ctx->device));
   const int64_t n = A->nrows;
   if (n == 0) return MFE_OK;
-  int* d_bc = nullptr;
-  unsigned char* flags = nullptr;
-  hipError_t e = hipMalloc(&d_bc, sizeof(int) * (size_t)(nbc > 0 ? nbc : 1));
-  if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc(bc rows)");
-  // flags[0, n): BC rows; flags[n, 2n): rows that may hold a BC column
-  e = hipMalloc(&flags, 2 * (size_t)n);
-  if (e != hipSuccess) {
-    hipFree(d_bc);
-    return hip_fail(ctx, e, "hipMalloc(bc flags)");
+  int* stage = nullptr;
+  if (nbc > 0) {
+    int sst = stage_buffer(ctx, reinterpret_cast<void**>(&stage));
+    if (sst) return sst;
   }
+  unsigned char* flags = nullptr;
+  // flags[0, n): BC rows; flags[n, 2n): rows that may hold a BC column
+  hipError_t e = hipMalloc(&flags, 2 * (size_t)n);
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc(bc flags)");
   unsigned char* const touch = flags + n;
   // the box stencil is structurally symmetric: only the BC rows' neighbours
   // can hold BC columns
   const bool box = A->box_elems[0] > 0;
   int st = MFE_OK;
   do {
-    if (nbc > 0) {
-      e = hipMemcpyAsync(d_bc, bc_rows, sizeof(int) * nbc, hipMemcpyHostToDevice, ctx->stream);
-      if (e != hipSuccess) { st = hip_fail(ctx, e, "bc rows H2D"); break; }
-    }
     e = hipMemsetAsync(flags, 0, 2 * (size_t)n, ctx->stream);
     if (e != hipSuccess) { st = hip_fail(ctx, e, "bc flags memset"); break; }
-    if (nbc > 0) {
-      bc_flag_kernel<<<(unsigned)cdiv(nbc, kBlock), kBlock, 0, ctx->stream>>>(nbc, d_bc, n, flags);
-      if ((e = hipGetLastError()) != hipSuccess) { st = hip_fail(ctx, e, "bc_flag_kernel"); break; }
+    // the kernels read the row list from the context's pinned buffer, a chunk
+    // at a time (no copy engine involved)
+    const int64_t chunk = (int64_t)(kStageBytes / sizeof(int));
+    for (int64_t off = 0; off < nbc && st == MFE_OK; off += chunk) {
+      const int64_t cnt = nbc - off < chunk ? nbc - off : chunk;
+      if (off > 0 && (e = hipStreamSynchronize(ctx->stream)) != hipSuccess) {
+        st = hip_fail(ctx, e, "bc rows");  // the previous chunk is still read
+        break;
+      }
+      std::memcpy(stage, bc_rows + off, sizeof(int) * (size_t)cnt);
+      bc_flag_kernel<<<(unsigned)cdiv(cnt, kBlock), kBlock, 0, ctx->stream>>>(cnt, stage, n, flags);
+      if (box)
+        bc_touch_kernel<<<(unsigned)cdiv(cnt, kBlock), kBlock, 0, ctx->stream>>>(
+            cnt, stage, n, A->row_offsets, A->cols, touch);
+      if ((e = hipGetLastError()) != hipSuccess) st = hip_fail(ctx, e, "bc_flag_kernel");
     }
+    if (st) break;
     csr_values_changed(A);
-    if (box) {
-      if (nbc > 0)
-        bc_touch_kernel<<<(unsigned)cdiv(nbc, kBlock), kBlock, 0, ctx->stream>>>(
-            nbc, d_bc, n, A->row_offsets, A->cols, touch);
+    if (box)
       dirichlet_touched_kernel<<<(unsigned)cdiv(n, kBlock), kBlock, 0, ctx->stream>>>(
           n, A->row_offsets, A->cols, A->vals, b->d, flags, touch, value);
-    } else if (A->max_block_nnz >= 0 && A->max_block_nnz <= kSpmvMaxNnz)
+    else if (A->max_block_nnz >= 0 && A->max_block_nnz <= kSpmvMaxNnz)
       dirichlet_tile_kernel<<<(unsigned)cdiv(n, kBlock), kBlock, 0, ctx->stream>>>(
           n, A->row_offsets, A->cols, A->vals, b->d, flags, value);
     else
@@ -569,7 +574,6 @@ int mfe_impose_dirichlet(mfe_ctx* ctx, double value, mfe_csr* A, mfe_vec* b, int
   } while (0);
   hipError_t se = hipStreamSynchronize(ctx->stream);
   if (st == MFE_OK && se != hipSuccess) st = hip_fail(ctx, se, "impose_dirichlet");
-  hipFree(d_bc);
   hipFree(flags);
   return st;
 }

@@ -38,6 +38,7 @@ struct mfe_ctx {
   size_t partials_cap = 0;
   double* d_scalars = nullptr;   // small device scalar area
   double* h_scalars = nullptr;   // pinned host mirror
+  void* h_stage = nullptr;       // pinned staging for small uploads (lazy, kStageBytes)
   mfe::CgWorkspace* cg = nullptr;
   mfe::MultiWorkspace* multi = nullptr;
   int cu_count = 0;
@@ -109,6 +110,15 @@ struct mfe_vec {
 namespace mfe {
 
 int fail(mfe_ctx* ctx, int status, const std::string& msg);
+// ctx's pinned host buffer of kStageBytes (allocated on first use), which
+// kernels can read directly; the caller waits for them before refilling it.
+constexpr size_t kStageBytes = 1 << 20;
+int stage_buffer(mfe_ctx* ctx, void** out);
+// Host <-> device copies through the pinned buffer done by a kernel that
+// reads or writes it (for the index arrays and counts the plans move; no copy
+// engine involved); on ctx's stream, complete on return (mfe_api.hip).
+int upload_bytes(mfe_ctx* ctx, void* dst, const void* src, size_t bytes);
+int download_bytes(mfe_ctx* ctx, void* dst, const void* src, size_t bytes);
 int hip_fail(mfe_ctx* ctx, hipError_t e, const char* what);
 // Recompute A->max_block_nnz on the device (synchronises the stream).
 int plan_spmv(mfe_ctx* ctx, mfe_csr* A);

@@ -678,11 +678,11 @@ static int build_sellx(mfe_ctx* ctx, mfe_csr* A) {
                                            nullptr, nullptr, nullptr);
   std::vector<int2> counts((size_t)nt);
   hipError_t e = hipGetLastError();
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(counts.data(), d_counts, sizeof(int2) * (size_t)nt, hipMemcpyDeviceToHost, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  int dst = e == hipSuccess ? download_bytes(ctx, counts.data(), d_counts, sizeof(int2) * (size_t)nt)
+                            : MFE_OK;
   (void)hipFree(d_counts);
   if (e != hipSuccess) return hip_fail(ctx, e, "sellx_plan_kernel count");
+  if (dst) return dst;
   std::vector<int> roff((size_t)nt + 1), nu((size_t)nt);
   int64_t runs = 0;
   int maxu = 0;
@@ -711,10 +711,10 @@ static int build_sellx(mfe_ctx* ctx, mfe_csr* A) {
   }
   MFE_CHECK(ctx, hipMemsetAsync(A->sellx_lcols + A->sell_slots, 0,
                                 sizeof(unsigned short) * kXTail, s));
-  MFE_CHECK(ctx, hipMemcpyAsync(A->sellx_roff, roff.data(), sizeof(int) * roff.size(),
-                                hipMemcpyHostToDevice, s));
-  MFE_CHECK(ctx, hipMemcpyAsync(A->sellx_nu, nu.data(), sizeof(int) * nu.size(),
-                                hipMemcpyHostToDevice, s));
+  int ust;
+  if ((ust = upload_bytes(ctx, A->sellx_roff, roff.data(), sizeof(int) * roff.size())) ||
+      (ust = upload_bytes(ctx, A->sellx_nu, nu.data(), sizeof(int) * nu.size())))
+    return ust;
   sellx_plan_kernel<<<nt, kBlock, 0, s>>>((int)n, ns, A->row_offsets, A->cols, A->sell_off,
                                            A->sell_len, 1, nullptr, A->sellx_roff, A->sellx_rcol,
                                            A->sellx_rpos, A->sellx_lcols, A->sellx_aff,
@@ -723,10 +723,12 @@ static int build_sellx(mfe_ctx* ctx, mfe_csr* A) {
   // slices with bases, and the stored entries of the others (plan figures)
   std::vector<unsigned char> aff((size_t)ns);
   std::vector<int> soff((size_t)ns + 1);
-  MFE_CHECK(ctx, hipMemcpyAsync(aff.data(), A->sellx_aff, (size_t)ns, hipMemcpyDeviceToHost, s));
-  MFE_CHECK(ctx, hipMemcpyAsync(soff.data(), A->sell_off, sizeof(int) * soff.size(),
-                                hipMemcpyDeviceToHost, s));
-  MFE_CHECK(ctx, hipStreamSynchronize(s));  // roff/nu host copies leave scope
+  {
+    int dst;
+    if ((dst = download_bytes(ctx, aff.data(), A->sellx_aff, (size_t)ns)) ||
+        (dst = download_bytes(ctx, soff.data(), A->sell_off, sizeof(int) * soff.size())))
+      return dst;
+  }
   int64_t naff = 0, nexp = 0;
   for (int i = 0; i < ns; ++i) {
     if (aff[(size_t)i]) ++naff;
@@ -770,9 +772,13 @@ static int ensure_plan(mfe_ctx* ctx, const mfe_csr* Ac, bool need64, bool need32
           (int)n, (int)ns, A->row_offsets, A->sell_len, d_width);
       MFE_LAUNCHED(ctx, "sell_len_kernel");
       std::vector<int> width((size_t)ns), off((size_t)ns + 1);
-      MFE_CHECK(ctx, hipMemcpyAsync(width.data(), d_width, sizeof(int) * (size_t)ns,
-                                    hipMemcpyDeviceToHost, s));
-      MFE_CHECK(ctx, hipStreamSynchronize(s));
+      {
+        const int dst = download_bytes(ctx, width.data(), d_width, sizeof(int) * (size_t)ns);
+        if (dst) {
+          (void)hipFree(d_width);
+          return dst;
+        }
+      }
       int64_t acc = 0;
       int maxw = 0;
       for (int64_t i = 0; i < ns; ++i) {
@@ -788,8 +794,8 @@ static int ensure_plan(mfe_ctx* ctx, const mfe_csr* Ac, bool need64, bool need32
            try_alloc(&A->sell_cols, sizeof(int) * (size_t)(A->sell_slots + kSellTail));
       if (ok) {
         MFE_CHECK(ctx, hipMemsetAsync(A->sell_cols + A->sell_slots, 0, sizeof(int) * kSellTail, s));
-        MFE_CHECK(ctx, hipMemcpyAsync(A->sell_off, off.data(), sizeof(int) * (size_t)(ns + 1),
-                                      hipMemcpyHostToDevice, s));
+        int ust = upload_bytes(ctx, A->sell_off, off.data(), sizeof(int) * (size_t)(ns + 1));
+        if (ust) return ust;
         sell_fill_kernel<int, int><<<(unsigned)ns, kSlice, 0, s>>>(
             (int)n, (int)ns, A->sell_off, A->sell_len, A->row_offsets, A->cols, A->sell_cols,
             true);
