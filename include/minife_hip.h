@@ -301,7 +301,12 @@ int mfe_spmm_select(mfe_ctx* ctx, const mfe_csr* A, int nrhs, int tile);
  * 1 = staged, the distinct X rows of 16 or 32 consecutive rows copied into
  * LDS first (available when no such row tile has more than 1024 non-zeros and
  * its X rows fit 64 KiB); 2 = SELL, values and columns read from A's SELL-64
- * copy (available unless A's format is CSR-only or a row is longer than 27).
+ * copy (available unless A's format is CSR-only or a row is longer than 27);
+ * 3 = SELL with grid-line slices (same availability as 2): a slice of 64
+ * consecutive nodes on grid lines, whose entries 3g, 3g+1, 3g+2 are the
+ * columns c-1, c, c+1, loads only the middle X row of each triple and takes
+ * the other two from the neighbouring rows' lanes (the same products, added
+ * in the same order: the bits of kind 2).
  * The tuner reports ms = +inf for an unavailable tile, and mfe_spmm rejects
  * it.  Outputs nullable. */
 int mfe_spmm_tile_info(int nrhs, int tile, int* cols_per_lane, int* unroll, int* rows_per_block,

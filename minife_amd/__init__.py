@@ -635,11 +635,12 @@ def spmm_tune(A: CSRMatrix, nrhs: int) -> dict:
     """Run the SpMM autotuner on A: {'tile': chosen id, 'ms': [...], 'tiles': [...]};
     ms is +inf for a tile that cannot run on A (a staged tile whose plan does
     not fit)."""
-    ms = np.zeros(16, np.float64)
+    tiles = spmm_tiles(nrhs)
+    ms = np.zeros(max(len(tiles), 1), np.float64)  # one entry per candidate tile
     tile, nt = ctypes.c_int(0), ctypes.c_int(0)
     A.ctx.check(lib().mfe_spmm_tune(A.ctx.h, A.h, int(nrhs), ctypes.byref(tile), _ptr(ms),
                                     ctypes.byref(nt)), "spmm_tune")
-    return {"tile": tile.value, "ms": ms[:nt.value].tolist(), "tiles": spmm_tiles(nrhs)}
+    return {"tile": tile.value, "ms": ms[:nt.value].tolist(), "tiles": tiles}
 
 
 def spmm_select(A: CSRMatrix, nrhs: int, tile: int) -> None:
@@ -657,7 +658,7 @@ def spmm_tiles(nrhs: int) -> list:
         if lib().mfe_spmm_tile_info(int(nrhs), i, ctypes.byref(c), ctypes.byref(u),
                                     ctypes.byref(r), ctypes.byref(st)) != 0:
             return tiles
-        kind = ("csr", "staged", "sell")[st.value]
+        kind = ("csr", "staged", "sell", "sell-lines")[st.value]
         tiles.append({"cols_per_lane": c.value, "unroll": u.value, "rows_per_block": r.value,
                       "kind": kind, "staged": kind == "staged"})
         i += 1

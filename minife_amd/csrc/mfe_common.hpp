@@ -98,6 +98,11 @@ struct mfe_csr {
   int64_t sellx_explicit = 0;              // stored entries of the other slices
   unsigned char* sellx_aff = nullptr;      // nslices: 1 = bases apply
   unsigned short* sellx_sbase = nullptr;   // nslices x kSellBaseStride bases (0 past the width)
+  // Per slice, whether its rows are 64 consecutive nodes of grid lines in the
+  // form the SpMM's line tiles use, and their columns (built for those tiles,
+  // mfe_multi.hip).
+  bool sell_lines_valid = false;
+  int* sell_lines = nullptr;
 };
 
 struct mfe_vec {

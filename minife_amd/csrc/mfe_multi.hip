@@ -25,27 +25,31 @@
 
 namespace mfe {
 
-constexpr int kSellTile = -1;  // SpmmTile::rows of a tile reading the SELL-64 copy
+constexpr int kSellTile = -1;      // SpmmTile::rows of a tile reading the SELL-64 copy
+constexpr int kSellLineTile = -2;  // the same, sharing X rows along grid lines
+constexpr int kLineStride = 16;    // ints per slice line record (count + 9 columns, 64 B)
 
 struct SpmmTile {
   int cpl, unroll;
   int rows;  // 0: CSR, direct gathers; 16 or 32: rows per workgroup of a staged tile;
-             // kSellTile: SELL-64 copy, direct gathers
+             // kSellTile: SELL-64 copy, direct gathers; kSellLineTile: SELL-64 copy,
+             // line slices taking two of each three X rows from neighbouring lanes
 };
 
 // Candidate tiles per K (index = tile id).
 static const SpmmTile kTiles16[] = {{1, 9, 0},  {2, 9, 0},  {4, 9, 0},  {8, 9, 0},  {2, 3, 0},
                                     {4, 3, 0},  {8, 3, 0},  {16, 3, 0}, {1, 9, 16}, {2, 9, 16},
                                     {1, 9, 32}, {2, 9, 32}, {4, 9, 32}, {2, 9, kSellTile},
-                                    {4, 9, kSellTile}, {8, 9, kSellTile}};
+                                    {4, 9, kSellTile}, {8, 9, kSellTile}, {4, 9, kSellLineTile},
+                                    {8, 9, kSellLineTile}};
 static const SpmmTile kTiles8[] = {{1, 9, 0},  {2, 9, 0},  {4, 9, 0},         {8, 9, 0},
                                    {2, 3, 0},  {4, 3, 0},  {8, 3, 0},         {1, 9, 16},
                                    {1, 9, 32}, {2, 9, 32}, {1, 9, kSellTile}, {2, 9, kSellTile},
-                                   {4, 9, kSellTile}};
+                                   {4, 9, kSellTile}, {2, 9, kSellLineTile}, {4, 9, kSellLineTile}};
 static const SpmmTile kTiles4[] = {{1, 9, 0},  {2, 9, 0},  {4, 9, 0},         {2, 3, 0},
                                    {4, 3, 0},  {1, 9, 16}, {1, 9, 32},        {1, 9, kSellTile},
-                                   {2, 9, kSellTile}};
-constexpr int kMaxTiles = 16;
+                                   {2, 9, kSellTile}, {1, 9, kSellLineTile}, {2, 9, kSellLineTile}};
+constexpr int kMaxTiles = 18;
 
 static int tile_count(int K) {
   return K == 16 ? (int)(sizeof(kTiles16) / sizeof(SpmmTile))
@@ -178,6 +182,22 @@ __device__ __forceinline__ double sum_same_column(double v) {
   return v;
 }
 
+// out[0..CPL) = src[0..CPL), in 16-byte loads when CPL is even.
+template <int CPL>
+__device__ __forceinline__ void load_x_piece(const double* __restrict__ src, double (&out)[CPL]) {
+  if constexpr (CPL % 2 == 0) {
+#pragma unroll
+    for (int q = 0; q < CPL; q += 2) {
+      const double2 d = *reinterpret_cast<const double2*>(src + q);
+      out[q] = d.x;
+      out[q + 1] = d.y;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) out[q] = src[q];
+  }
+}
+
 // Y = A X; with partials non-null also the workgroup partials of Y[:,k].P[:,k]
 // (partials[block * K + k]), P = X here (the CG's p.Ap).
 template <int K, int CPL, int U>
@@ -210,20 +230,7 @@ __global__ __launch_bounds__(kBlock) void spmm_kernel(int n, const int* __restri
       }
       double xv[U][CPL];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const double* src = X + (int64_t)c[u] * K + k0;
-        if constexpr (CPL % 2 == 0) {
-#pragma unroll
-          for (int q = 0; q < CPL; q += 2) {
-            const double2 d = *reinterpret_cast<const double2*>(src + q);
-            xv[u][q] = d.x;
-            xv[u][q + 1] = d.y;
-          }
-        } else {
-#pragma unroll
-          for (int q = 0; q < CPL; ++q) xv[u][q] = src[q];
-        }
-      }
+      for (int u = 0; u < U; ++u) load_x_piece<CPL>(X + (int64_t)c[u] * K + k0, xv[u]);
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (j0 + u < len) {
@@ -412,6 +419,7 @@ struct SpmmArgs {
   const unsigned char* slen;
   const int* scols;
   const double* svals;
+  const int* lines;  // per-slice line records (null: line tiles unavailable)
 };
 
 static bool stage_ok(const SpmmPlan* p, int K) {
@@ -422,10 +430,13 @@ static bool stage_ok(const SpmmPlan* p, int K) {
 // its row at sell_pos (the L lanes of a row read the same entry, the rows of a
 // wave consecutive entries); RPB rows per workgroup cover whole or parts of
 // slices; rows past the last slice return at once, rows past n add nothing.
-template <int K, int CPL, int U>
+// With LINES, a slice that sell_lines_kernel recorded as a line slice takes
+// the line path below instead; its products and their order are the same.
+template <int K, int CPL, int U, bool LINES>
 __global__ __launch_bounds__(kBlock) void spmm_sell_kernel(
     int n, int nslices, const int* __restrict__ soff, const unsigned char* __restrict__ slen,
-    const int* __restrict__ scols, const double* __restrict__ svals, const double* __restrict__ X,
+    const int* __restrict__ scols, const double* __restrict__ svals,
+    const int* __restrict__ lines, const double* __restrict__ X,
     double* __restrict__ Y, double* __restrict__ partials, const int* __restrict__ done) {
   if (done && *done) return;
   constexpr int L = K / CPL;
@@ -438,7 +449,55 @@ __global__ __launch_bounds__(kBlock) void spmm_sell_kernel(
   double acc[CPL];
 #pragma unroll
   for (int q = 0; q < CPL; ++q) acc[q] = 0.0;
-  if (live) {
+  // a wave's rows lie in one slice: its line record is wave-uniform
+  const int* rec = LINES && live
+                       ? lines + (int64_t)__builtin_amdgcn_readfirstlane(slice) * kLineStride
+                       : nullptr;
+  const int ntr = rec ? rec[0] : 0;
+  if (ntr > 0) {
+    // Line slice: its 64 rows all have width w = 3 ntr, and slots 3g, 3g+1,
+    // 3g+2 of row r hold columns c_g + r - 1, c_g + r, c_g + r + 1
+    // (consecutive nodes of grid lines; c_g = rec[1 + g]).  Slot 3g's X row
+    // of row r is slot 3g+1's X row of row r-1, and slot 3g+2's that of row
+    // r+1: each lane loads its row's middle X row and takes the other two
+    // from the lanes L below and above it; the lanes of the wave's first and
+    // last rows load their missing one.  No column loads: the X loads go out
+    // with the value loads.
+    static_assert(U % 3 == 0, "line path unrolls whole slot triples");
+    constexpr int G = U / 3;
+    const int w = 3 * ntr;
+    const int64_t o64 = (int64_t)soff[slice] * kWave;
+    const int hl = t % kWave;
+    const bool head = hl < L, tail = hl >= kWave - L;
+    for (int g0 = 0; g0 < ntr; g0 += G) {
+      double v[U];
+      double xm[G][CPL], xe[G][CPL];
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const int gg = min(g0 + g, ntr - 1);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) v[3 * g + m] = svals[sell_pos(o64, 3 * gg + m, w, lane)];
+        const int c = rec[1 + gg] + lane;
+        load_x_piece<CPL>(X + (int64_t)c * K + k0, xm[g]);
+        if (head || tail) load_x_piece<CPL>(X + (int64_t)(head ? c - 1 : c + 1) * K + k0, xe[g]);
+      }
+#pragma unroll
+      for (int g = 0; g < G; ++g)
+        if (g0 + g < ntr) {
+#pragma unroll
+          for (int q = 0; q < CPL; ++q) {
+            const double lo = __shfl_up(xm[g][q], L, kWave);
+            const double hi = __shfl_down(xm[g][q], L, kWave);
+            acc[q] += v[3 * g] * (head ? xe[g][q] : lo);
+            acc[q] += v[3 * g + 1] * xm[g][q];
+            acc[q] += v[3 * g + 2] * (tail ? xe[g][q] : hi);
+          }
+        }
+    }
+    double* dst = Y + (int64_t)row * K + k0;  // every row of a line slice is < n
+#pragma unroll
+    for (int q = 0; q < CPL; ++q) dst[q] = acc[q];
+  } else if (live) {
     const int o = soff[slice], w = soff[slice + 1] - o;
     const int len = slen[row];
     const int64_t o64 = (int64_t)o * kWave;
@@ -453,20 +512,7 @@ __global__ __launch_bounds__(kBlock) void spmm_sell_kernel(
       }
       double xv[U][CPL];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const double* src = X + (int64_t)c[u] * K + k0;
-        if constexpr (CPL % 2 == 0) {
-#pragma unroll
-          for (int q = 0; q < CPL; q += 2) {
-            const double2 d = *reinterpret_cast<const double2*>(src + q);
-            xv[u][q] = d.x;
-            xv[u][q + 1] = d.y;
-          }
-        } else {
-#pragma unroll
-          for (int q = 0; q < CPL; ++q) xv[u][q] = src[q];
-        }
-      }
+      for (int u = 0; u < U; ++u) load_x_piece<CPL>(X + (int64_t)c[u] * K + k0, xv[u]);
 #pragma unroll
       for (int u = 0; u < U; ++u)
         if (j0 + u < len) {
@@ -502,6 +548,53 @@ __global__ __launch_bounds__(kBlock) void spmm_sell_kernel(
   }
 }
 
+// Line record of slice s, lines[s * kLineStride ...]: [0] = w / 3 when
+// slice s can take spmm_sell_kernel's line path, else 0; then [1 + g] = slot
+// 3g+1's column in the slice's first row.  The path applies when the 64 rows
+// (all < n) have the slice's width w, a positive multiple of 3; slot j's
+// column is slot j's column of the first row + lane; and slots 3g+1 and 3g+2
+// hold slot 3g's column + 1 and + 2.  One wave per slice.
+__global__ __launch_bounds__(kBlock) void sell_lines_kernel(int n, int nslices,
+                                                            const int* __restrict__ soff,
+                                                            const unsigned char* __restrict__ slen,
+                                                            const int* __restrict__ scols,
+                                                            int* __restrict__ lines) {
+  const int slice = (int)blockIdx.x * (kBlock / kWave) + (int)threadIdx.x / kWave;
+  const int lane = (int)threadIdx.x % kWave;
+  if (slice >= nslices) return;  // whole waves
+  const int o = soff[slice], w = soff[slice + 1] - o;
+  const int64_t row = (int64_t)slice * kWave + lane;
+  const int64_t o64 = (int64_t)o * kWave;
+  int* const rec = lines + (int64_t)slice * kLineStride;
+  bool ok = w > 0 && w % 3 == 0 && w / 3 < kLineStride && row < n && slen[row] == w;
+  int prev = 0;
+  for (int j = 0; j < w; ++j) {
+    const int c = scols[sell_pos(o64, j, w, lane)];
+    const int c0 = __shfl(c, 0, kWave);
+    ok = ok && c == c0 + lane && (j % 3 == 0 || c == prev + 1);
+    if (lane == 0 && j % 3 == 1 && 1 + j / 3 < kLineStride) rec[1 + j / 3] = c;
+    prev = c;
+  }
+  const int all = __all(ok ? 1 : 0);
+  if (lane == 0) rec[0] = all ? w / 3 : 0;
+}
+
+// The line flags of A's SELL-64 copy (built once per structure).
+static int ensure_lines(mfe_ctx* ctx, const mfe_csr* Ac) {
+  mfe_csr* A = const_cast<mfe_csr*>(Ac);
+  if (!A->sell_ok || A->sell_lines_valid) return MFE_OK;
+  const int ns = (int)A->sell_nslices;
+  if (!A->sell_lines)
+    MFE_CHECK(ctx, hipMalloc(&A->sell_lines, sizeof(int) * kLineStride * (size_t)std::max(ns, 1)));
+  if (ns > 0) {
+    sell_lines_kernel<<<(unsigned)cdiv(ns, kBlock / kWave), kBlock, 0, ctx->stream>>>(
+        (int)A->nrows, ns, A->sell_off, A->sell_len, A->sell_cols, A->sell_lines);
+    MFE_LAUNCHED(ctx, "sell_lines_kernel");
+  }
+  A->sell_lines_valid = true;
+  return MFE_OK;
+}
+
 // Staged shapes need whole waves and at most 1024 threads: 64 <= R (K / CPL).
 template <int K, int CPL, int U, int R>
 static hipError_t launch_staged(const SpmmArgs& a, size_t lds, const double* X, double* Y,
@@ -522,22 +615,27 @@ static hipError_t launch_spmm(int tile, const SpmmArgs& a, const double* X, doub
                               double* partials, const int* done, hipStream_t s, int* blocks_out) {
   const SpmmTile T = tile_of(K, tile);
   const int n = a.n;
-  if (T.rows == kSellTile) {
-    if (!a.svals) return hipErrorInvalidValue;
+  if (T.rows == kSellTile || T.rows == kSellLineTile) {
+    const bool lines = T.rows == kSellLineTile;
+    if (!a.svals || (lines && !a.lines)) return hipErrorInvalidValue;
     const int rpb = kBlock / (K / T.cpl);
     const unsigned g = (unsigned)cdiv((int64_t)a.nslices * kWave, rpb);
     if (blocks_out) *blocks_out = (int)g;
     if (n == 0) return hipSuccess;
-#define MFE_SELL_CASE(C, UU)                                                                 \
-    if (T.cpl == C && T.unroll == UU) {                                                     \
-      spmm_sell_kernel<K, (C <= K ? C : K), UU><<<g, kBlock, 0, s>>>(                       \
-          n, a.nslices, a.soff, a.slen, a.scols, a.svals, X, Y, partials, done);           \
-      return hipGetLastError();                                                             \
+#define MFE_SELL_CASE(C, UU, LN)                                                                \
+    if (T.cpl == C && T.unroll == UU && lines == LN) {                                         \
+      spmm_sell_kernel<K, (C <= K ? C : K), UU, LN><<<g, kBlock, 0, s>>>(                      \
+          n, a.nslices, a.soff, a.slen, a.scols, a.svals, a.lines, X, Y, partials, done);     \
+      return hipGetLastError();                                                                \
     }
-    MFE_SELL_CASE(1, 9)
-    MFE_SELL_CASE(2, 9)
-    MFE_SELL_CASE(4, 9)
-    MFE_SELL_CASE(8, 9)
+    MFE_SELL_CASE(1, 9, false)
+    MFE_SELL_CASE(2, 9, false)
+    MFE_SELL_CASE(4, 9, false)
+    MFE_SELL_CASE(8, 9, false)
+    MFE_SELL_CASE(1, 9, true)
+    MFE_SELL_CASE(2, 9, true)
+    MFE_SELL_CASE(4, 9, true)
+    MFE_SELL_CASE(8, 9, true)
 #undef MFE_SELL_CASE
     return hipErrorInvalidValue;
   }
@@ -596,6 +694,7 @@ static int k_slot(int K) { return K == 16 ? 2 : K == 8 ? 1 : 0; }
 static bool tile_available(int K, int tile, const SpmmArgs& a) {
   const int R = tile_of(K, tile).rows;
   if (R == kSellTile) return a.svals != nullptr && a.n > 0;
+  if (R == kSellLineTile) return a.svals != nullptr && a.lines != nullptr && a.n > 0;
   return R == 0 || stage_ok(a.plans[plan_slot(R)], K);
 }
 
@@ -915,7 +1014,7 @@ static int ensure_plan(mfe_ctx* ctx, const mfe_csr* A, int R) {
 // and A's SELL-64 copy when it is current.
 static SpmmArgs spmm_args(const mfe_csr* A, const MultiWorkspace* w) {
   SpmmArgs a{(int)A->nrows, A->row_offsets, A->cols, A->vals, {nullptr, nullptr},
-             0, nullptr, nullptr, nullptr, nullptr};
+             0, nullptr, nullptr, nullptr, nullptr, nullptr};
   if (w)
     for (int i = 0; i < 2; ++i)
       if (w->plans[i].built && w->plans[i].A == A && w->plans[i].gen == A->plan_gen)
@@ -926,6 +1025,7 @@ static SpmmArgs spmm_args(const mfe_csr* A, const MultiWorkspace* w) {
     a.slen = A->sell_len;
     a.scols = A->sell_cols;
     a.svals = A->sell_vals;
+    if (A->sell_lines_valid) a.lines = A->sell_lines;
   }
   return a;
 }
@@ -1030,8 +1130,8 @@ static bool k_ok(int K) { return K == 4 || K == 8 || K == 16; }
 static int ensure_tuned(mfe_ctx* ctx, const mfe_csr* A, int K, MultiWorkspace* w) {
   const int slot = k_slot(K);
   int st;
-  if ((st = sell_prepare(ctx, A)) || (st = ensure_plan(ctx, A, 16)) ||
-      (st = ensure_plan(ctx, A, 32)))
+  if ((st = sell_prepare(ctx, A)) || (st = ensure_lines(ctx, A)) ||
+      (st = ensure_plan(ctx, A, 16)) || (st = ensure_plan(ctx, A, 32)))
     return st;
   if (w->tun.A == A && w->tun.gen == A->plan_gen && w->tun.tile[slot] >= 0) return MFE_OK;
   if (w->tun.A != A || w->tun.gen != A->plan_gen) {
@@ -1073,8 +1173,8 @@ int mfe_spmm_select(mfe_ctx* ctx, const mfe_csr* A, int nrhs, int tile) {
   MultiWorkspace* w = nullptr;
   int st;
   if ((st = multi_prepare(ctx, A->nrows, nrhs, &w))) return st;
-  if ((st = sell_prepare(ctx, A)) || (st = ensure_plan(ctx, A, 16)) ||
-      (st = ensure_plan(ctx, A, 32)))
+  if ((st = sell_prepare(ctx, A)) || (st = ensure_lines(ctx, A)) ||
+      (st = ensure_plan(ctx, A, 16)) || (st = ensure_plan(ctx, A, 32)))
     return st;
   if (!tile_available(nrhs, tile, spmm_args(A, w)))
     return fail(ctx, MFE_BAD_ARG, "spmm_select: staged tile unavailable for this matrix");
@@ -1094,7 +1194,7 @@ int mfe_spmm_tile_info(int nrhs, int tile, int* cols_per_lane, int* unroll, int*
   if (cols_per_lane) *cols_per_lane = T.cpl;
   if (unroll) *unroll = T.unroll;
   if (rows_per_block) *rows_per_block = T.rows > 0 ? T.rows : kBlock / (nrhs / T.cpl);
-  if (kind) *kind = T.rows > 0 ? 1 : T.rows == kSellTile ? 2 : 0;
+  if (kind) *kind = T.rows > 0 ? 1 : T.rows == kSellTile ? 2 : T.rows == kSellLineTile ? 3 : 0;
   return MFE_OK;
 }
 
@@ -1118,8 +1218,8 @@ int mfe_spmm(mfe_ctx* ctx, const mfe_csr* A, const mfe_vec* X, mfe_vec* Y, int n
       return fail(ctx, MFE_BAD_ARG,
                   "spmm: staged tile unavailable for this matrix (a row tile has more than "
                   "1024 non-zeros or its distinct X rows exceed 64 KiB)");
-  } else if (R == kSellTile) {
-    if ((st = sell_prepare(ctx, A))) return st;
+  } else if (R == kSellTile || R == kSellLineTile) {
+    if ((st = sell_prepare(ctx, A)) || (st = ensure_lines(ctx, A))) return st;
     if (!(A->sell_ok && A->sell_vals_valid) || A->nrows == 0)
       return fail(ctx, MFE_BAD_ARG,
                   "spmm: SELL tile unavailable for this matrix (CSR format, or a row longer "

@@ -630,6 +630,9 @@ void sell_release(mfe_csr* A) {
   hipFree(A->sell_cols);
   hipFree(A->sell_vals);
   hipFree(A->sell_vals32);
+  hipFree(A->sell_lines);
+  A->sell_lines = nullptr;
+  A->sell_lines_valid = false;
   A->sell_off = nullptr;
   A->sell_len = nullptr;
   A->sell_cols = nullptr;

@@ -1,8 +1,10 @@
 """GPU many right-hand sides: blocked SpMM and simultaneous CG.
 
-Checks: every SpMM tile (CSR, SELL-64 and LDS-staged) gives, in column k,
-the single-vector SpMV's bits (the oracle's matvec) for nrhs = 4, 8, 16 on
-stencil and ragged matrices; each tile is available exactly when its
+Checks: every SpMM tile (CSR, SELL-64, SELL-64 with grid-line slices and
+LDS-staged) gives, in column k, the single-vector SpMV's bits (the oracle's
+matvec) for nrhs = 4, 8, 16 on stencil and ragged matrices, the stencil cases
+including one whose 101-node grid lines give line slices (counted here with
+numpy); each tile is available exactly when its
 conditions hold (computed here with numpy: SELL rows of at most 27 entries;
 staged row tiles of at most 1024 non-zeros whose distinct X rows fit 60 KiB)
 and an unavailable one is refused; the autotuner picks one of its timed
@@ -34,12 +36,34 @@ def mfe():
     return minife_amd
 
 
+def line_slices(ro, cols, n):
+    """Slices of 64 rows that the line tiles treat as grid lines (numpy
+    restatement of the library's condition): all 64 rows < n with the same
+    length w, a positive multiple of 3; entry j of row r0 + l is entry j of
+    row r0 plus l; entries 3g+1 and 3g+2 of a row are entry 3g plus 1 and 2."""
+    count = 0
+    for r0 in range(0, n - 63, 64):
+        lens = np.diff(ro[r0:r0 + 65])
+        w = int(lens[0])
+        if w == 0 or w % 3 or np.any(lens != w):
+            continue
+        block = np.stack([cols[ro[r0 + l]:ro[r0 + l] + w] for l in range(64)])
+        ok = np.all(block == block[0] + np.arange(64)[:, None])
+        ok = ok and np.all((block[:, 1:] - block[:, :-1])[:, [j - 1 for j in range(1, w) if j % 3]] == 1)
+        count += bool(ok)
+    return count
+
+
 def test_spmm_every_tile_bit_exact(mfe, oracle_mod):
-    for dims in [(17, 3, 5), (12, 10, 9)]:
+    for dims in [(17, 3, 5), (12, 10, 9), (100, 4, 3)]:
         P = oracle_mod.build_problem(*dims)
         Ao = P["A"]
         A = mfe.CSRMatrix.from_host(Ao.row_offsets, Ao.cols, Ao.vals)
         n = Ao.nrows
+        nlines = line_slices(Ao.row_offsets, Ao.cols, n)
+        say(f"spmm {dims}: {nlines} line slices of {(n + 63) // 64}")
+        if dims[0] >= 64:
+            assert nlines > 0
         for nrhs in (4, 8, 16):
             X = np.random.default_rng(nrhs).standard_normal((n, nrhs))
             want = np.stack([oracle_mod.matvec(Ao, np.ascontiguousarray(X[:, k]))
@@ -60,7 +84,7 @@ def tile_expected_available(shape, ro, cols, n, nrhs):
     library's conditions)."""
     if shape["kind"] == "csr":
         return True
-    if shape["kind"] == "sell":
+    if shape["kind"] in ("sell", "sell-lines"):
         return n > 0 and int(np.diff(ro).max()) <= 27
     R = shape["rows_per_block"]
     maxu = 0
