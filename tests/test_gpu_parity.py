@@ -509,6 +509,34 @@ def test_dirichlet_on_uploaded_matrices(mfe, ctx, oracle_mod, maxlen):
         assert bits_equal(b.to_numpy(), bo)
 
 
+def test_dirichlet_bc_rows_beyond_one_staging_chunk(mfe, ctx, oracle_mod):
+    """impose_dirichlet reads the BC row list through a pinned buffer of
+    262,144 rows at a time, waiting for the kernels before each refill:
+    550,000 BC rows of a 700,000-row five-point band (random values) take
+    three fills, then 50,000 more rows one.  Values and right-hand side equal
+    the oracle's bit for bit after each call (value 0, then 1)."""
+    say("case dirichlet bc rows beyond one staging chunk")
+    n = 700_000
+    rng = np.random.default_rng(31)
+    r = np.repeat(np.arange(n), 5)
+    c = r + np.tile(np.arange(-2, 3), n)
+    keep = (c >= 0) & (c < n)
+    cols = c[keep].astype(np.int32)
+    ro = np.concatenate([[0], np.cumsum(np.bincount(r[keep], minlength=n))]).astype(np.int32)
+    vals = rng.standard_normal(cols.size)
+    bh = rng.standard_normal(n)
+    bc = np.sort(rng.choice(n, size=600_000, replace=False)).astype(np.int32)
+    Ao = oracle_mod.CSR(np.arange(n, dtype=np.int32), ro, cols, vals.copy())
+    bo = bh.copy()
+    A = mfe.CSRMatrix.from_host(ro, cols, vals, ctx=ctx)
+    b = mfe.Vector.from_numpy(bh, ctx)
+    for value, rows in ((0.0, bc[:550_000]), (1.0, bc[550_000:])):
+        mfe.impose_dirichlet(value, A, b, 0, 0, 0, rows)
+        oracle_mod.impose_dirichlet(value, Ao, bo, rows)
+        assert bits_equal(A.to_host(rows=False, row_offsets=False, cols=False)["vals"], Ao.vals)
+        assert bits_equal(b.to_numpy(), bo)
+
+
 def test_staged_x_plan_bitmap_and_sort_tiles(mfe, ctx, oracle_mod):
     """The plan lists a tile's distinct columns with an LDS bitmap when they
     span at most 2^19 columns and by sorting otherwise.  700,000 rows of a
